@@ -1,0 +1,152 @@
+// Host-side view of a device-resident graph and the launch helpers shared by the HIP
+// translation units.  Everything allocated here lives in HBM for the life of the handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "device.hpp"
+#include "eeg_internal.hpp"
+
+#define EEG_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::eeg::fail(e_ == hipErrorOutOfMemory ? EEG_ERR_OOM : EEG_ERR_DEVICE, \
+                               "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                               __FILE__, __LINE__);                                       \
+    } while (0)
+
+#define EEG_TRY(expr)                      \
+    do {                                   \
+        eeg_status s_ = (expr);            \
+        if (s_ != EEG_OK) return s_;       \
+    } while (0)
+
+namespace eeg {
+
+// Owning device allocation.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    eeg_status alloc(size_t n) {
+        if (p && bytes >= n && n > 0) return EEG_OK;
+        release();
+        if (n == 0) n = 8;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(EEG_ERR_OOM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+        return EEG_OK;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Compressed sparse rows of one edge direction.
+struct Csr {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    DevMem row;  // u64[n+1]
+    DevMem col;  // u32[m]
+    DevMem w;    // u32[m] (empty if unweighted)
+    bool built = false;
+    const uint64_t* rowp() const { return row.as<const uint64_t>(); }
+    const uint32_t* colp() const { return col.as<const uint32_t>(); }
+    const uint32_t* wp() const { return w.p ? w.as<const uint32_t>() : nullptr; }
+};
+
+// Tigr virtual-node tables for one K, run-length encoded: the vertices with at least one
+// edge (vlist, ascending) and the exclusive prefix of ceil(deg/K) over them (vscan, nz+1
+// entries).  Virtual node t belongs to vlist[i] with vscan[i] <= t < vscan[i+1] and covers
+// that vertex's edges [row + (t - vscan[i]) * K, +K).
+struct Tigr {
+    uint32_t K = 0;
+    uint32_t nz = 0;
+    uint64_t P = 0;
+    DevMem vlist;  // u32[nz]
+    DevMem vscan;  // u32[nz+1]
+    bool built = false;
+};
+
+// Reusable per-graph traversal state (one query at a time per handle).
+struct Work {
+    DevMem lab;         // u32[n] (or f32 ranks)
+    DevMem q[2], s[2];  // frontier queues: entry vertex ids and vnode prefix
+    DevMem stamp;       // u32[n] iteration stamps for queue dedup
+    DevMem flag;        // u32[n] iteration stamps for flag-scan frontiers
+    DevMem vnpush;      // u32[P] last label pushed by each virtual node (async dd)
+    DevMem ctl;         // Ctl
+    DevMem fx[2], fc[2], fr, fxc;  // PR: ranks, contributions, residual, Kahan terms
+    DevMem spill;       // PR: per-tile partial sums
+    DevMem red;         // PR: per-workgroup residual partials
+    uint32_t epoch = 1; // iteration stamps already used on stamp/flag
+    Ctl* host_ctl = nullptr;  // pinned mirrors of Ctl (two, for overlapped polling)
+    std::vector<hipEvent_t> events;  // pool: one or two per launch
+    eeg_status event(size_t i, hipEvent_t* out);
+    ~Work();
+};
+
+}  // namespace eeg
+
+struct eeg_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool weighted = false;
+    uint32_t n = 0;
+    uint64_t m = 0;
+    eeg::Csr out, sym, in;
+    eeg::Tigr tout[4], tsym[4], tin[4];  // index log2(K) - 2
+    eeg::Work work;
+    uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
+    std::mutex mu;
+    ~eeg_graph();
+};
+
+namespace eeg {
+
+inline int k_index(uint32_t K) {
+    switch (K) {
+        case 4: return 0;
+        case 8: return 1;
+        case 16: return 2;
+        case 32: return 3;
+    }
+    return -1;
+}
+
+// Graph construction (build.hip).
+eeg_status build_csr_from_device_edges(eeg_graph* g, const uint32_t* d_src, const uint32_t* d_dst,
+                                       const uint32_t* d_w, uint64_t m);
+eeg_status ensure_tigr(eeg_graph* g, Csr& c, Tigr* table, uint32_t K);
+eeg_status ensure_sym(eeg_graph* g);
+eeg_status ensure_in(eeg_graph* g);
+eeg_status export_tigr(eeg_graph* g, uint32_t K, uint64_t* node_pointer, uint32_t* edge_list,
+                       uint32_t* parts, uint64_t* num_parts);
+eeg_status generate_rmat_device(uint32_t scale, uint32_t ef, double a, double b, double c,
+                                uint64_t seed, uint32_t wmin, uint32_t wmax, uint32_t* d_src,
+                                uint32_t* d_dst, uint32_t* d_w, hipStream_t stream);
+
+// Traversals (relax.hip, pagerank.hip).
+eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                         eeg_stats* st);
+eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                        eeg_stats* st);
+
+}  // namespace eeg

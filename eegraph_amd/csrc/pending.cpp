@@ -1,0 +1,19 @@
+// Entry points whose implementation has not landed yet.
+#include "eeg_internal.hpp"
+
+extern "C" eeg_status eeg_run_batch(eeg_graph*, uint32_t, const uint32_t*, uint32_t, const eeg_opts*, void*, eeg_stats*) {
+    return eeg::fail(EEG_ERR_UNSUPPORTED, "batched queries not built yet");
+}
+extern "C" eeg_status eeg_run_packed(eeg_graph* const*, uint32_t, uint32_t, const uint32_t*, const eeg_opts*, void* const*, eeg_stats*) {
+    return eeg::fail(EEG_ERR_UNSUPPORTED, "packed graphs not built yet");
+}
+extern "C" eeg_status eeg_autotune(eeg_graph*, uint32_t, const uint32_t*, uint32_t, eeg_opts*, double*) {
+    return eeg::fail(EEG_ERR_UNSUPPORTED, "autotuner not built yet");
+}
+extern "C" eeg_status eeg_server_create(eeg_graph*, const eeg_server_cfg*, eeg_server**) {
+    return eeg::fail(EEG_ERR_UNSUPPORTED, "server not built yet");
+}
+extern "C" eeg_status eeg_server_submit(eeg_server*, uint32_t, uint32_t, uint64_t*) { return EEG_ERR_UNSUPPORTED; }
+extern "C" eeg_status eeg_server_wait(eeg_server*, uint64_t, void*, double, eeg_query_result*) { return EEG_ERR_UNSUPPORTED; }
+extern "C" eeg_status eeg_server_cancel(eeg_server*, uint64_t) { return EEG_ERR_UNSUPPORTED; }
+extern "C" eeg_status eeg_server_destroy(eeg_server*) { return EEG_ERR_UNSUPPORTED; }

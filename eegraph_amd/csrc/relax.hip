@@ -1,0 +1,615 @@
+// Push relaxation over Tigr virtual nodes for BFS / SSSP / SSWP / CC, and the host loop that
+// drives it.  Replaces the reference kernels {bfs,sssp,sswp,cc}::{async,sync}_push_{td,dd}
+// (src/bfs.cu:27-273, src/sssp.cu:26-299, src/sswp.cu:26-271, src/cc.cu:26-271) and the
+// iteration loops of src/eegraph.cu (e.g. :2113-2236).
+//
+// Work decomposition.  A frontier is a list of entries (vertices with out-edges) plus the
+// exclusive prefix S of their virtual-node counts ceil(deg/K).  The grid walks the virtual
+// nodes in tiles of TILE_EDGES/K; every workgroup owns a contiguous run of tiles, stages the
+// tile's entries (vertex, row offset, degree, label) in LDS, maps each virtual node to its
+// entry by an LDS binary search, and then relaxes one edge per lane with K lanes per
+// virtual node, so a wavefront reads 64 consecutive destination slots of a hub.
+//
+// Frontier modes (eeg_variant):
+//   M_TD    every virtual node every sweep; stop after a sweep without improvement.
+//   M_FLAGS per-vertex iteration stamps: active iff flag[v] == iter; improvements stamp iter+1.
+//   M_ASYNC per-virtual-node "last label pushed": a virtual node runs when its vertex label is
+//           better than what it pushed last, labels in place (no sibling clears, unlike D5).
+//   M_QUEUE compacted queue: improved vertices are deduplicated with a stamp, gathered in LDS
+//           with wavefront ballots and appended with one packed 64-bit atomic per tile, which
+//           reserves queue slots and virtual-node prefix together.
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace eeg {
+namespace {
+
+enum Mode : int { M_TD = 0, M_FLAGS = 1, M_ASYNC = 2, M_QUEUE = 3 };
+
+struct RelaxArgs {
+    const uint64_t* row;
+    const uint32_t* col;
+    const uint32_t* wgt;
+    uint32_t* lab;
+    const uint32_t* qin;  // frontier entries (vertex ids)
+    const uint32_t* sin;  // their virtual-node prefix
+    uint32_t* qout;
+    uint32_t* sout;
+    uint32_t* stamp;
+    uint32_t* flag;
+    uint32_t* vnpush;
+    Ctl* ctl;
+    uint32_t n_static;  // entries / vnodes when qslot_in < 0
+    uint32_t P_static;
+    int qslot_in;       // Ctl::qpack slot holding the input size, or -1
+    int qslot_out;
+    uint32_t cap_out;
+    uint32_t it;        // 1-based iteration of this traversal
+    uint32_t stampv;    // stamp value of this iteration
+    uint32_t count_stats;
+};
+
+template <int ALG, int K, int MODE>
+__global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
+    using R = Rule<ALG>;
+    constexpr int VPR = WG / K;             // virtual nodes per round
+    constexpr int TV = TILE_EDGES / K;      // virtual nodes per tile
+    constexpr int ROUNDS = TILE_EDGES / WG;
+    constexpr int EMAX = TV + 2;            // staged entries: every entry has >= 1 vnode
+    constexpr int QCAP = (MODE == M_QUEUE) ? TILE_EDGES : 1;
+
+    __shared__ uint32_t sS[EMAX];
+    __shared__ uint64_t sRow[EMAX];
+    __shared__ uint32_t sDeg[EMAX];
+    __shared__ uint32_t sLab[EMAX];
+    __shared__ uint16_t sOwn[TV];
+    __shared__ uint32_t sQ[QCAP];
+    __shared__ uint32_t sQn[QCAP];
+    __shared__ uint32_t sMisc[8];
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    uint32_t nE, T;
+    if (a.qslot_in >= 0) {
+        const unsigned long long pk = ctl->qpack[a.qslot_in];
+        nE = (uint32_t)(pk & 0xffffffffull);
+        T = (uint32_t)(pk >> 32);
+    } else {
+        nE = a.n_static;
+        T = a.P_static;
+    }
+    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    if (MODE == M_QUEUE) {
+        if (nE == 0 || T == 0) {
+            if (lead) ctl->done = 1;
+            return;
+        }
+    } else if (a.it > 1 && ctl->changed[(a.it - 1) & 3] == 0) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    if (lead) {
+        ctl->iters = a.it;
+        if (MODE == M_QUEUE) ctl->qpack[(a.it + 1) % 3] = 0;
+        else ctl->changed[(a.it + 1) & 3] = 0;
+    }
+    if (T == 0) return;
+
+    uint64_t st_ent = 0, st_vn = 0, st_edge = 0, st_upd = 0;
+    bool changed = false;
+    const uint32_t ntiles = (uint32_t)(((uint64_t)T + TV - 1) / TV);
+    const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
+    const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+    const int grp = threadIdx.x / K, lj = threadIdx.x % K;
+
+    if (c0 < c1) {
+        uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+        for (uint32_t tile = c0; tile < c1; ++tile) {
+            const uint32_t t0 = tile * TV;
+            const uint32_t t1 = min(t0 + (uint32_t)TV, T);
+            for (int k = threadIdx.x; k < EMAX; k += WG) {
+                const uint64_t i = (uint64_t)e_start + k;
+                sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
+            }
+            if (threadIdx.x == 0) {
+                sMisc[0] = 1;
+                sMisc[1] = 1;
+                sMisc[2] = 0;
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < EMAX; k += WG) {
+                const uint32_t cur = sS[k];
+                const uint32_t nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+                if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;  // entries in the tile
+                if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;          // ... and up to t1
+            }
+            __syncthreads();
+            const int nent = (int)sMisc[0];
+            const int nnext = (int)sMisc[1];
+            for (int k = threadIdx.x; k < nent; k += WG) {
+                const uint32_t v = a.qin[e_start + k];
+                const uint64_t r0 = a.row[v], r1 = a.row[v + 1];
+                const uint32_t lab = a.lab[v];
+                uint32_t deg = (uint32_t)(r1 - r0);
+                if (MODE == M_FLAGS && a.flag[v] != a.stampv) deg = 0;
+                if (!R::usable(lab)) deg = 0;
+                sRow[k] = r0;
+                sDeg[k] = deg;
+                sLab[k] = lab;
+                st_ent += 1;
+            }
+            for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
+                const uint32_t t = t0 + j;
+                int lo = 0, hi = nent;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sS[mid] <= t) lo = mid; else hi = mid;
+                }
+                sOwn[j] = (uint16_t)lo;
+            }
+            __syncthreads();
+
+#pragma unroll 2
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int slot = r * VPR + grp;
+                const uint32_t t = t0 + slot;
+                bool upd = false;
+                uint32_t dv = 0;
+                if (t < t1) {
+                    const int k = sOwn[slot];
+                    const uint32_t eoff = (t - sS[k]) * K + lj;
+                    const uint32_t deg = sDeg[k];
+                    const uint32_t lab = sLab[k];
+                    bool act = deg > 0;
+                    if (MODE == M_ASYNC && act) {
+                        act = R::better(lab, a.vnpush[t]);
+                        if (act && lj == 0) a.vnpush[t] = lab;
+                    }
+                    if (act && lj == 0) ++st_vn;
+                    if (act && eoff < deg) {
+                        const uint64_t e = sRow[k] + eoff;
+                        const uint32_t d = a.col[e];
+                        const uint32_t w = R::weighted ? a.wgt[e] : 1u;
+                        const uint32_t nl = R::cand(lab, w);
+                        ++st_edge;
+                        if (R::better(nl, a.lab[d])) {
+                            const uint32_t old = R::apply(&a.lab[d], nl);
+                            if (R::better(nl, old)) {
+                                upd = true;
+                                dv = d;
+                                ++st_upd;
+                            }
+                        }
+                    }
+                }
+                if (MODE == M_QUEUE) {
+                    bool cand = false;
+                    uint32_t nv = 0;
+                    if (upd) {
+                        const uint64_t r0 = a.row[dv], r1 = a.row[dv + 1];
+                        if (r1 > r0 && atomicExch(&a.stamp[dv], a.stampv) != a.stampv) {
+                            cand = true;
+                            nv = (uint32_t)((r1 - r0 + K - 1) / K);
+                        }
+                    }
+                    const uint64_t mask = __ballot(cand);
+                    if (mask) {
+                        const int leader = __ffsll((unsigned long long)mask) - 1;
+                        uint32_t base = 0;
+                        if (lane_id() == leader) base = atomicAdd(&sMisc[2], (uint32_t)__popcll(mask));
+                        base = __shfl(base, leader, WAVE);
+                        if (cand) {
+                            const uint32_t pos = base + (uint32_t)__popcll(mask & lanemask_lt());
+                            sQ[pos] = dv;
+                            sQn[pos] = nv;
+                        }
+                    }
+                } else {
+                    if (upd) {
+                        changed = true;
+                        if (MODE == M_FLAGS) a.flag[dv] = a.stampv + 1;
+                    }
+                }
+            }
+
+            if (MODE == M_QUEUE) {
+                __syncthreads();
+                const uint32_t cnt = sMisc[2];
+                if (cnt > 0) {
+                    const uint32_t per = (cnt + WG - 1) / WG;
+                    const uint32_t b0 = threadIdx.x * per;
+                    uint64_t loc = 0;
+                    for (uint32_t i = 0; i < per; ++i)
+                        if (b0 + i < cnt) loc += sQn[b0 + i];
+                    uint64_t tot;
+                    uint64_t pre = block_excl_scan<uint64_t>(loc, sScan, &tot);
+                    for (uint32_t i = 0; i < per; ++i)
+                        if (b0 + i < cnt) {
+                            const uint32_t x = sQn[b0 + i];
+                            sQn[b0 + i] = (uint32_t)pre;
+                            pre += x;
+                        }
+                    if (threadIdx.x == 0) {
+                        const unsigned long long add = ((unsigned long long)tot << 32) | cnt;
+                        const unsigned long long old = atomicAdd(&ctl->qpack[a.qslot_out], add);
+                        sMisc[3] = (uint32_t)(old & 0xffffffffull);
+                        sMisc[4] = (uint32_t)(old >> 32);
+                    }
+                    __syncthreads();
+                    const uint32_t bq = sMisc[3], bv = sMisc[4];
+                    if ((uint64_t)bq + cnt <= a.cap_out) {
+                        for (uint32_t i = threadIdx.x; i < cnt; i += WG) {
+                            a.qout[bq + i] = sQ[i];
+                            a.sout[bq + i] = bv + sQn[i];
+                        }
+                    } else if (threadIdx.x == 0) {
+                        atomicOr(&ctl->err, 1u);
+                    }
+                }
+            }
+            __syncthreads();
+            e_start += (uint32_t)(nnext - 1);
+        }
+    }
+    if (MODE != M_QUEUE) {
+        if (__ballot(changed) && lane_id() == 0) ctl->changed[a.it & 3] = 1;
+    }
+    if (a.count_stats) {
+        st_ent = wave_sum(st_ent);
+        st_vn = wave_sum(st_vn);
+        st_edge = wave_sum(st_edge);
+        st_upd = wave_sum(st_upd);
+        if (lane_id() == 0 && (st_ent | st_vn | st_edge | st_upd)) {
+            atomicAdd(&ctl->stat[0], (unsigned long long)st_ent);
+            atomicAdd(&ctl->stat[1], (unsigned long long)st_vn);
+            atomicAdd(&ctl->stat[2], (unsigned long long)st_edge);
+            atomicAdd(&ctl->stat[3], (unsigned long long)st_upd);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- initialisation
+// Initial labels (src/eegraph.cu:38-49 BFS/SSSP, :1549-1560 SSWP, :544-553 CC): the host
+// picks the values; CC passes identity = 1 (label = own id).  Written as a select rather than
+// a switch on the algorithm: hipcc 7.2 lowered that switch for gfx950 with the default arm
+// unreachable (the CC label was never written).
+__global__ void k_init_labels(uint32_t* __restrict__ lab, uint32_t n, uint32_t fill,
+                              uint32_t at_source, uint32_t source, uint32_t identity) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (v == source) ? at_source : fill;
+        lab[v] = identity ? (uint32_t)v : x;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint64_t n, uint32_t val) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = val;
+}
+
+// Clears the control block and seeds the first frontier with `source` (when seed != 0).
+__global__ void k_init_ctl(Ctl* ctl, const uint64_t* row, uint32_t K, int seed, uint32_t source,
+                           uint32_t* q0, uint32_t* s0) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < 3; ++i) ctl->qpack[i] = 0;
+    for (int i = 0; i < 4; ++i) {
+        ctl->changed[i] = 0;
+        ctl->stat[i] = 0;
+        ctl->residual[i] = 0;
+    }
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->err = 0;
+    ctl->mass = 0;
+    if (seed) {
+        const uint64_t d = row[source + 1] - row[source];
+        if (d > 0) {
+            q0[0] = source;
+            s0[0] = 0;
+            ctl->qpack[0] = ((unsigned long long)((d + K - 1) / K) << 32) | 1ull;
+        }
+    }
+}
+
+unsigned grid_for(uint64_t work, unsigned cap) {
+    uint64_t g = (work + WG - 1) / WG;
+    if (g == 0) g = 1;
+    return (unsigned)(g > cap ? cap : g);
+}
+
+using KernelFn = void (*)(RelaxArgs);
+
+template <int ALG, int K>
+KernelFn pick_mode(int mode) {
+    switch (mode) {
+        case M_TD: return k_relax<ALG, K, M_TD>;
+        case M_FLAGS: return k_relax<ALG, K, M_FLAGS>;
+        case M_ASYNC: return k_relax<ALG, K, M_ASYNC>;
+        default: return k_relax<ALG, K, M_QUEUE>;
+    }
+}
+template <int ALG>
+KernelFn pick_k(uint32_t K, int mode) {
+    switch (K) {
+        case 4: return pick_mode<ALG, 4>(mode);
+        case 8: return pick_mode<ALG, 8>(mode);
+        case 16: return pick_mode<ALG, 16>(mode);
+        default: return pick_mode<ALG, 32>(mode);
+    }
+}
+KernelFn pick_kernel(int alg, uint32_t K, int mode) {
+    switch (alg) {
+        case A_BFS: return pick_k<A_BFS>(K, mode);
+        case A_SSSP: return pick_k<A_SSSP>(K, mode);
+        case A_SSWP: return pick_k<A_SSWP>(K, mode);
+        default: return pick_k<A_CC>(K, mode);
+    }
+}
+
+int cu_count(int device) {
+    static int cached[64] = {0};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c <= 0)
+        c = 256;
+    if (device >= 0 && device < 64) cached[device] = c;
+    return c;
+}
+
+}  // namespace
+
+// Memory the traversal state needs beyond the graph (used by admission control too).
+size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
+    size_t b = (size_t)n * 4 + sizeof(Ctl);
+    if (mode == M_QUEUE) b += (size_t)n * 4 * 5;
+    if (mode == M_FLAGS) b += (size_t)n * 4;
+    if (mode == M_ASYNC) b += (size_t)P * 4;
+    return b;
+}
+
+eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                         eeg_stats* st) {
+    const auto t_host0 = std::chrono::steady_clock::now();
+    const uint32_t K = o.K ? o.K : 8;
+    if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
+    if (o.block && o.block != WG) return fail(EEG_ERR_INVALID_ARG, "block must be %d", WG);
+    int mode;
+    switch (o.variant) {
+        case EEG_ASYNC_PUSH_TD:
+        case EEG_SYNC_PUSH_TD: mode = M_TD; break;
+        case EEG_ASYNC_PUSH_DD: mode = M_ASYNC; break;
+        case EEG_SYNC_PUSH_DD: mode = (o.frontier == EEG_FRONTIER_FLAGS) ? M_FLAGS : M_QUEUE; break;
+        default: return fail(EEG_ERR_INVALID_ARG, "unknown variant %u", o.variant);
+    }
+    const int alg = (int)algo;  // EEG_BFS..EEG_CC map onto A_BFS..A_CC
+    if ((alg == A_SSSP || alg == A_SSWP) && !g->weighted)
+        return fail(EEG_ERR_INVALID_ARG, "SSSP/SSWP need a weighted graph");
+    if (alg == A_CC) EEG_TRY(ensure_sym(g));
+    Csr& c = (alg == A_CC) ? g->sym : g->out;
+    Tigr* tt = (alg == A_CC) ? g->tsym : g->tout;
+    EEG_TRY(ensure_tigr(g, c, tt, K));
+    const Tigr& t = tt[k_index(K)];
+    const uint32_t n = g->n;
+    hipStream_t s = g->stream;
+    Work& w = g->work;
+
+    EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
+    EEG_TRY(w.ctl.alloc(sizeof(Ctl)));
+    if (!w.host_ctl) EEG_HIP(hipHostMalloc((void**)&w.host_ctl, 2 * sizeof(Ctl), hipHostMallocDefault));
+    if (mode == M_QUEUE) {
+        for (int i = 0; i < 2; ++i) {
+            EEG_TRY(w.q[i].alloc((size_t)n * 4 + 4));
+            EEG_TRY(w.s[i].alloc((size_t)n * 4 + 4));
+        }
+    }
+    const uint32_t max_iters = o.max_iters ? o.max_iters : (n + 2);
+    if (mode == M_QUEUE && !w.stamp.p) {
+        EEG_TRY(w.stamp.alloc((size_t)n * 4 + 4));
+        EEG_HIP(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, s));
+    }
+    if (mode == M_FLAGS && !w.flag.p) {
+        EEG_TRY(w.flag.alloc((size_t)n * 4 + 4));
+        EEG_HIP(hipMemsetAsync(w.flag.p, 0, w.flag.bytes, s));
+    }
+    if ((uint64_t)w.epoch + max_iters + 200 >= 0xfffffff0ull) {  // stamps would wrap: reset
+        if (w.stamp.p) EEG_HIP(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, s));
+        if (w.flag.p) EEG_HIP(hipMemsetAsync(w.flag.p, 0, w.flag.bytes, s));
+        w.epoch = 1;
+    }
+    if (mode == M_ASYNC) EEG_TRY(w.vnpush.alloc((size_t)t.P * 4 + 4));
+
+    // ---- labels and control block
+    {
+        const uint32_t fill = (alg == A_SSWP) ? 0u : INF;
+        const uint32_t at_src = (alg == A_SSWP) ? INF : 0u;
+        k_init_labels<<<grid_for(n, 8192), WG, 0, s>>>(w.lab.as<uint32_t>(), n, fill, at_src,
+                                                       source, alg == A_CC ? 1u : 0u);
+    }
+    EEG_HIP(hipGetLastError());
+    if (mode == M_ASYNC) {
+        const uint32_t never = (alg == A_SSWP) ? 0u : 0xffffffffu;
+        k_fill_u32<<<grid_for(t.P, 8192), WG, 0, s>>>(w.vnpush.as<uint32_t>(), t.P, never);
+        EEG_HIP(hipGetLastError());
+    }
+    const uint32_t base = w.epoch;  // iteration it uses stamp base + it
+    const bool seeded = (alg != A_CC);
+    Ctl* ctl = w.ctl.as<Ctl>();
+    k_init_ctl<<<1, 64, 0, s>>>(ctl, c.rowp(), K, (seeded && mode == M_QUEUE) ? 1 : 0, source,
+                                mode == M_QUEUE ? w.q[0].as<uint32_t>() : nullptr,
+                                mode == M_QUEUE ? w.s[0].as<uint32_t>() : nullptr);
+    EEG_HIP(hipGetLastError());
+    if (mode == M_FLAGS) {
+        if (seeded) k_fill_u32<<<1, 64, 0, s>>>(w.flag.as<uint32_t>() + source, 1, base + 1);
+        else k_fill_u32<<<grid_for(n, 8192), WG, 0, s>>>(w.flag.as<uint32_t>(), n, base + 1);
+        EEG_HIP(hipGetLastError());
+    }
+
+    // ---- launch geometry: a resident grid that walks tiles
+    const int cus = cu_count(g->device);
+    const uint32_t TV = TILE_EDGES / K;
+    const uint64_t static_tiles = (t.P + TV - 1) / TV;
+    const unsigned grid_static =
+        (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * 8));
+    const unsigned grid_queue = (unsigned)cus * 4;
+    KernelFn kfn = pick_kernel(alg, K, mode);
+
+    RelaxArgs ra{};
+    ra.row = c.rowp();
+    ra.col = c.colp();
+    ra.wgt = c.wp();
+    ra.lab = w.lab.as<uint32_t>();
+    ra.stamp = w.stamp.as<uint32_t>();
+    ra.flag = w.flag.as<uint32_t>();
+    ra.vnpush = w.vnpush.as<uint32_t>();
+    ra.ctl = ctl;
+    ra.cap_out = n;
+    ra.n_static = t.nz;
+    ra.P_static = (uint32_t)t.P;
+    ra.count_stats = (o.count_work || o.collect_activity) ? 1u : 0u;
+
+    // Event pool: per launch L (it = L+1) an "after" event at 2L+1 and, with timing, a
+    // "before" event at 2L; event 0 of the pool tail marks the traversal start.
+    hipEvent_t ev_start;
+    EEG_TRY(w.event(0, &ev_start));
+    auto launch = [&](uint32_t it) -> eeg_status {
+        ra.it = it;
+        ra.stampv = base + it;
+        unsigned grid = grid_static;
+        if (mode == M_QUEUE) {
+            if (it == 1 && !seeded) {
+                ra.qin = t.vlist.as<uint32_t>();
+                ra.sin = t.vscan.as<uint32_t>();
+                ra.qslot_in = -1;
+            } else {
+                ra.qin = w.q[(it - 1) & 1].as<uint32_t>();
+                ra.sin = w.s[(it - 1) & 1].as<uint32_t>();
+                ra.qslot_in = (int)((it - 1) % 3);
+                grid = grid_queue;
+            }
+            ra.qout = w.q[it & 1].as<uint32_t>();
+            ra.sout = w.s[it & 1].as<uint32_t>();
+            ra.qslot_out = (int)(it % 3);
+        } else {
+            ra.qin = t.vlist.as<uint32_t>();
+            ra.sin = t.vscan.as<uint32_t>();
+            ra.qslot_in = -1;
+        }
+        hipEvent_t eb, ea;
+        if (o.timing) {
+            EEG_TRY(w.event(2 * (size_t)it - 1, &eb));
+            EEG_HIP(hipEventRecord(eb, s));
+        }
+        void* args[] = {&ra};
+        EEG_HIP(hipLaunchKernel((const void*)kfn, dim3(grid), dim3(WG), args, 0, s));
+        EEG_TRY(w.event(2 * (size_t)it, &ea));
+        EEG_HIP(hipEventRecord(ea, s));
+        return EEG_OK;
+    };
+
+    EEG_HIP(hipEventRecord(ev_start, s));
+    uint32_t it = 1;
+    eeg_status rc = EEG_OK;
+    std::vector<uint32_t> activity;
+    if (o.collect_activity) {
+        // one launch at a time so every frontier size can be read back
+        while (true) {
+            EEG_TRY(launch(it));
+            EEG_HIP(hipMemcpyAsync(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            EEG_HIP(hipStreamSynchronize(s));
+            const Ctl& h = w.host_ctl[0];
+            if (h.err || h.done) break;
+            if (mode == M_QUEUE) activity.push_back((uint32_t)(h.qpack[(it - 1) % 3] & 0xffffffffull));
+            else activity.push_back((uint32_t)h.stat[3]);
+            if (++it > max_iters) {
+                rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
+                break;
+            }
+        }
+    } else {
+        // Chunks of launches; the next chunk is queued before the previous chunk's control
+        // block is inspected, so the GPU never waits for the host between chunks.
+        uint32_t chunk = 2;
+        hipEvent_t poll[2];
+        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
+        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        auto submit = [&](int slot) -> eeg_status {
+            for (uint32_t k = 0; k < chunk; ++k, ++it) EEG_TRY(launch(it));
+            EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            EEG_HIP(hipEventRecord(poll[slot], s));
+            chunk = std::min<uint32_t>(chunk * 2, 8);
+            return EEG_OK;
+        };
+        int cur = 0;
+        rc = submit(0);
+        while (rc == EEG_OK) {
+            if (it <= max_iters + 8) rc = submit(cur ^ 1);
+            if (rc != EEG_OK) break;
+            EEG_HIP(hipEventSynchronize(poll[cur]));
+            const Ctl& h = w.host_ctl[cur];
+            if (h.err || h.done) break;
+            if (it > max_iters + 8) {
+                rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
+                break;
+            }
+            cur ^= 1;
+        }
+        (void)hipEventDestroy(poll[0]);
+        (void)hipEventDestroy(poll[1]);
+    }
+    EEG_HIP(hipStreamSynchronize(s));
+    EEG_HIP(hipMemcpy(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const Ctl hc = w.host_ctl[0];
+    if (rc == EEG_OK && hc.err) rc = fail(EEG_ERR_DEVICE, "frontier queue overflow (err=%u)", hc.err);
+    if (rc == EEG_OK && !hc.done) rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
+    w.epoch = base + it + 2;
+
+    if (st) {
+        hipEvent_t ev_end;
+        EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(hc.iters, 1), &ev_end));
+        float ms = 0;
+        EEG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end));
+        st->iterations = hc.iters;
+        st->launches = it - 1;
+        st->ms_traversal = ms;
+        st->edges_relaxed = hc.stat[2];
+        st->vnodes_processed = hc.stat[1];
+        st->vertices_touched = hc.stat[0];
+        st->label_updates = hc.stat[3];
+        const double ce = (alg == A_SSSP || alg == A_SSWP) ? 12.0 : 8.0;
+        double fb = 0;
+        if (mode == M_QUEUE) fb = 8.0 * (double)hc.stat[3];
+        if (mode == M_FLAGS) fb = 4.0 * (double)hc.stat[0];
+        if (mode == M_ASYNC) fb = 4.0 * (double)hc.stat[1];
+        st->bytes_alg = 12.0 * hc.stat[0] + ce * hc.stat[2] + 4.0 * hc.stat[3] + fb;
+        double ksum = 0;
+        uint32_t kn = 0;
+        if (o.timing) {
+            for (uint32_t i = 1; i <= hc.iters; ++i) {
+                hipEvent_t eb, ea;
+                EEG_TRY(w.event(2 * (size_t)i - 1, &eb));
+                EEG_TRY(w.event(2 * (size_t)i, &ea));
+                float km = 0;
+                EEG_HIP(hipEventElapsedTime(&km, eb, ea));
+                ksum += km;
+                ++kn;
+            }
+        }
+        st->ms_kernel_sum = ksum;
+        st->kernels_timed = kn;
+        st->activity_len = 0;
+        if (st->activity && st->activity_cap) {
+            const uint32_t L = (uint32_t)std::min<size_t>(activity.size(), st->activity_cap);
+            for (uint32_t i = 0; i < L; ++i) st->activity[i] = activity[i];
+            st->activity_len = L;
+        }
+        st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    }
+    return rc;
+}
+
+}  // namespace eeg

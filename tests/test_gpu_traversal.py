@@ -1,0 +1,160 @@
+"""GPU parity of the push traversals (BFS/SSSP/SSWP/CC) against the CPU oracle.
+
+Integer labels must match bit for bit: the textbook oracles (queue BFS, Dijkstra, max-min
+Dijkstra, union-find) give the unique fixpoint every correct push schedule reaches.
+"""
+import numpy as np
+import pytest
+
+import eegraph_amd as eg
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+ALG = {"bfs": O.BFS, "sssp": O.SSSP, "sswp": O.SSWP, "cc": O.CC}
+VARIANTS = ["async_push_td", "async_push_dd", "sync_push_td", "sync_push_dd"]
+
+
+@pytest.fixture(scope="module")
+def rmat14():
+    """Config 1 graph: RMAT scale 14, EF 16, seed 1, weights U[1,64]."""
+    s, d, w = eg.gen_rmat(14, 16, seed=1, wmin=1, wmax=64)
+    n = 1 << 14
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    yield n, s, d, w, g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def refs14(rmat14):
+    n, s, d, w, g = rmat14
+    src = g.info()["argmax_out_degree"]
+    return src, {a: O.reference_labels(ALG[a], n, s, d, w, src) for a in ALG}
+
+
+def test_info_argmax(rmat14):
+    n, s, d, w, g = rmat14
+    deg = np.bincount(s, minlength=n)
+    info = g.info()
+    assert info["n"] == n and info["m"] == len(s)
+    assert info["max_out_degree"] == deg.max()
+    assert info["argmax_out_degree"] == int(np.argmax(deg))  # lowest id among maxima
+    assert info["nonzero_out"] == int((deg > 0).sum())
+
+
+def test_csr_matches_stable_counting_sort(rmat14):
+    n, s, d, w, g = rmat14
+    row, col, wo = g.csr_export()
+    orow, ocol, ow = O.csr(n, s, d, w)
+    assert np.array_equal(row, orow)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(wo, ow)
+
+
+@pytest.mark.parametrize("K", [4, 8, 16, 32])
+def test_tigr_layout_matches_oracle(rmat14, K):
+    n, s, d, w, g = rmat14
+    np_, el, parts = g.tigr_export(K)
+    onp, oel, oparts = O.tigr_layout(n, s, d, w, K)
+    assert np.array_equal(np_, onp)
+    assert np.array_equal(el, oel)
+    assert np.array_equal(parts, oparts)
+
+
+def test_sssp_sync_push_config1(rmat14, refs14):
+    """Config 1: SSSP sync push, K=8, source = argmax out-degree, vs serial Dijkstra."""
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run("sssp", src, variant="sync_push_dd", K=8)
+    assert np.array_equal(lab, refs["sssp"])
+    assert st["iterations"] > 0
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("K", [4, 8, 16, 32])
+def test_all_variants_bit_exact(rmat14, refs14, alg, variant, K):
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run(alg, src, variant=variant, K=K)
+    assert np.array_equal(lab, refs[alg]), f"{alg} {variant} K={K}"
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+def test_flag_scan_frontier(rmat14, refs14, alg):
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run(alg, src, variant="sync_push_dd", K=8, frontier="flags")
+    assert np.array_equal(lab, refs[alg])
+
+
+def test_device_rmat_equals_host_rmat():
+    s, d, w = eg.gen_rmat(12, 8, seed=5, wmin=1, wmax=64)
+    gh = eg.Graph.from_edges(s, d, w, n=1 << 12)
+    gd = eg.Graph.rmat(12, 8, seed=5, wmin=1, wmax=64)
+    for a, b in zip(gh.csr_export(), gd.csr_export()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+def test_grid_with_shortcuts(alg):
+    s, d, w = eg.gen_grid(96, 80, 0.01, seed=3, wmin=1, wmax=1000)
+    n = 96 * 80
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    ref = O.reference_labels(ALG[alg], n, s, d, w, 0)
+    for variant in VARIANTS:
+        for frontier in ("compact", "flags"):
+            lab, st = g.run(alg, 0, variant=variant, K=4, frontier=frontier)
+            assert np.array_equal(lab, ref), f"{alg} {variant} {frontier}"
+
+
+def test_edge_cases_tiny_graphs():
+    """Isolated source, self-loops, duplicates, a single vertex, and brute-force closure."""
+    rng = np.random.default_rng(11)
+    for trial in range(12):
+        n = int(rng.integers(1, 40))
+        m = int(rng.integers(0, 3 * n + 1))
+        s = rng.integers(0, n, m).astype(np.uint32)
+        d = rng.integers(0, n, m).astype(np.uint32)
+        w = rng.integers(1, 65, m).astype(np.uint32)
+        if m:
+            s[0] = d[0] = 0  # a self-loop on vertex 0
+        g = eg.Graph.from_edges(s, d, w, n=n)
+        for alg in ALG:
+            src = int(rng.integers(0, n))
+            ref = O.brute(ALG[alg], n, s, d, w, src)
+            for variant in VARIANTS:
+                lab, _ = g.run(alg, src, variant=variant, K=4)
+                assert np.array_equal(lab, ref), f"trial {trial} {alg} {variant}"
+        g.close()
+
+
+def test_source_out_of_range_is_rejected(rmat14):
+    n, s, d, w, g = rmat14
+    with pytest.raises(eg.EEGError) as e:
+        g.run("bfs", n)
+    assert e.value.status == 1
+
+
+def test_work_counters_and_timing(rmat14, refs14):
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run("sssp", src, variant="sync_push_dd", K=8, count_work=True, timing=True)
+    assert np.array_equal(lab, refs["sssp"])
+    assert st["edges_relaxed"] >= np.count_nonzero(refs["sssp"][s] < O.INF)
+    assert st["label_updates"] >= np.count_nonzero(refs["sssp"] < O.INF) - 1
+    assert st["kernels_timed"] == st["iterations"] and st["ms_kernel_sum"] > 0
+
+
+def test_symmetrised_and_in_edge_csr(rmat14):
+    """CC runs on every edge in both directions; PageRank gathers over in-edges.  Both are
+    built on the device by a stable sort; the oracle builds them by counting sort."""
+    n, s, d, w, g = rmat14
+    orow, ocol, _ = O.csr(n, s, d)
+    src_exp = np.repeat(np.arange(n, dtype=np.uint32), np.diff(orow.astype(np.int64)))
+    row, col = g.csr_export_derived("sym")
+    srow, scol, _ = O.csr(n, np.concatenate([src_exp, ocol]), np.concatenate([ocol, src_exp]))
+    assert np.array_equal(row, srow) and np.array_equal(col, scol)
+    row, col = g.csr_export_derived("in")
+    irow, icol, _ = O.csr(n, ocol, src_exp)
+    assert np.array_equal(row, irow) and np.array_equal(col, icol)
