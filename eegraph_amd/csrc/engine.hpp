@@ -93,7 +93,7 @@ struct Work {
     DevMem flag;        // u32[n] iteration stamps for flag-scan frontiers
     DevMem vnpush;      // u32[P] last label pushed by each virtual node (async dd)
     DevMem ctl;         // Ctl
-    DevMem fx[2], fc[2], fr, fxc;  // PR: ranks, contributions, residual, Kahan terms
+    DevMem fx[2], fl[2], fc[2];  // PR: ranks as (hi, lo) pairs and contributions x/outdeg
     DevMem spill;       // PR: per-tile partial sums
     DevMem red;         // PR: per-workgroup residual partials
     uint32_t epoch = 1; // iteration stamps already used on stamp/flag

@@ -1,8 +1,516 @@
-// PageRank / personalised PageRank (Jacobi with compensated sums, residual push).
+// PageRank and personalised PageRank.
+//
+// The reference pushes float deltas with atomicAdd (src/pr.cu:3-288): the order of additions
+// changes from run to run and every part of a multi-part vertex consumes the same delta
+// (D6).  Here:
+//   sync variants  - Jacobi iteration x' = (1-d)/N + d * sum_{u->v} x_u/outdeg(u), x0 = 1/N,
+//                    gathered per destination over the in-edge CSR with Tigr virtual nodes
+//                    (K in-edges each, K lanes, one edge per lane).  Each virtual node reduces
+//                    its K contributions with a fixed TwoSum tree, each vertex adds its virtual
+//                    nodes in order with compensation (Kahan/TwoSum), vertices that span tiles
+//                    are finished by a fix-up pass in tile order.  Results are deterministic.
+//                    The L1 residual is accumulated in FP64 and reduced in a fixed order.
+//   async variants - residual (delta) push: x_v += r_v (compensated), d*r_v/deg pushed to the
+//                    out-neighbours' residual with float atomics, until every residual is below
+//                    a threshold.  Personalised PageRank always runs this way from a one-hot
+//                    residual (1-d) at the source, so it only touches the source's neighbourhood.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
 #include "engine.hpp"
 
 namespace eeg {
-eeg_status run_pagerank(eeg_graph*, uint32_t, uint32_t, const eeg_opts&, eeg_stats*) {
-    return fail(EEG_ERR_UNSUPPORTED, "PageRank not built yet");
+namespace {
+
+// ---------------------------------------------------------------- compensated arithmetic
+struct Comp {  // value s with the rounding error e that s + e represents exactly
+    float s, e;
+};
+__device__ __forceinline__ Comp two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float err = (a - (s - bb)) + (b - bb);
+    return Comp{s, err};
 }
+__device__ __forceinline__ Comp comp_add(Comp a, Comp b) {
+    const Comp t = two_sum(a.s, b.s);
+    return Comp{t.s, t.e + (a.e + b.e)};
+}
+__device__ __forceinline__ float comp_value(Comp a) { return a.s + a.e; }
+
+// Butterfly over a group of K lanes: every lane ends with the same (s, e), because each step
+// combines the same two operands on both partner lanes (addition is commutative and the
+// TwoSum error is exact).
+template <int K>
+__device__ __forceinline__ Comp group_reduce(Comp c) {
+#pragma unroll
+    for (int off = 1; off < K; off <<= 1) {
+        Comp o;
+        o.s = __shfl_xor(c.s, off, WAVE);
+        o.e = __shfl_xor(c.e, off, WAVE);
+        c = comp_add(c, o);
+    }
+    return c;
+}
+
+struct PullArgs {
+    const uint64_t* row;     // in-edge CSR
+    const uint32_t* col;     // sources of in-edges
+    const uint32_t* vlist;   // vertices with in-edges
+    const uint32_t* vscan;   // their virtual-node prefix
+    uint32_t nz;
+    uint32_t P;
+    const float* c_old;      // contributions x_u / outdeg(u)
+    const float* x_old;      // ranks as compensated pairs: x = hi + lo
+    const float* xl_old;
+    float* x_new;
+    float* xl_new;
+    float* c_new;
+    const uint64_t* out_row; // out-degrees for the new contributions
+    float2* spill;           // per tile: head and tail partials
+    double* res_part;        // per workgroup FP64 residual
+    Ctl* ctl;
+    float base;              // (1-d)/N
+    float damping;
+    uint32_t it;
+};
+
+// New rank base + d * sum kept as an FP32 pair (hi, lo): the product d*s is split exactly with
+// an FMA, base + d*s with TwoSum, and the small terms folded into lo.  The pair carries ~46
+// bits, so the iteration settles on an exact fixed point instead of cycling by one FP32 ulp.
+__device__ __forceinline__ void rank_pair(float base, float d, Comp sum, float* hi, float* lo) {
+    const float p = d * sum.s;
+    const float pe = fmaf(d, sum.s, -p);
+    const Comp q = two_sum(base, p);
+    const float small = q.e + (pe + d * sum.e);
+    const float h = q.s + small;            // fast two-sum: |q.s| >= |small|
+    *hi = h;
+    *lo = small - (h - q.s);
+}
+
+__device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Comp sum, double& res) {
+    float hi, lo;
+    rank_pair(a.base, a.damping, sum, &hi, &lo);
+    const double xo = (double)a.x_old[v] + (double)a.xl_old[v];
+    a.x_new[v] = hi;
+    a.xl_new[v] = lo;
+    const uint64_t d = a.out_row[v + 1] - a.out_row[v];
+    a.c_new[v] = d ? (hi + lo) / (float)d : 0.0f;
+    res += fabs(((double)hi + (double)lo) - xo);
+}
+
+template <int K>
+__global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
+    constexpr int VPR = WG / K;
+    constexpr int TV = TILE_EDGES / K;
+    constexpr int ROUNDS = TILE_EDGES / WG;
+    constexpr int EMAX = TV + 2;
+    __shared__ uint32_t sS[EMAX];
+    __shared__ uint64_t sRow[EMAX];
+    __shared__ uint32_t sDeg[EMAX];
+    __shared__ uint16_t sOwn[TV];
+    __shared__ float2 sPart[TV];
+    __shared__ uint32_t sMisc[4];
+    __shared__ double sRed[WG / WAVE];
+
+    if (a.ctl->done) return;
+    double res = 0.0;
+    const uint32_t T = a.P, nE = a.nz;
+    const uint32_t ntiles = (T + TV - 1) / TV;
+    const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
+    const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+    const int grp = threadIdx.x / K, lj = threadIdx.x % K;
+    if (c0 < c1) {
+        uint32_t e_start = wg_search(a.vscan, nE, c0 * TV);
+        for (uint32_t tile = c0; tile < c1; ++tile) {
+            const uint32_t t0 = tile * TV;
+            const uint32_t t1 = min(t0 + (uint32_t)TV, T);
+            for (int k = threadIdx.x; k < EMAX; k += WG) {
+                const uint64_t i = (uint64_t)e_start + k;
+                sS[k] = (i <= nE) ? a.vscan[i] : 0xffffffffu;  // vscan[nE] = P closes the last
+            }
+            if (threadIdx.x == 0) {
+                sMisc[0] = 1;
+                sMisc[1] = 1;
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < EMAX; k += WG) {
+                const uint32_t cur = sS[k];
+                const uint32_t nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+                if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;  // entries in the tile
+                if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;          // ... and up to t1
+            }
+            __syncthreads();
+            const int nent = (int)sMisc[0];
+            const int nnext = (int)sMisc[1];
+            for (int k = threadIdx.x; k < nent; k += WG) {
+                const uint32_t v = a.vlist[e_start + k];
+                const uint64_t r0 = a.row[v];
+                sRow[k] = r0;
+                sDeg[k] = (uint32_t)(a.row[v + 1] - r0);
+            }
+            for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
+                const uint32_t t = t0 + j;
+                int lo = 0, hi = nent;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sS[mid] <= t) lo = mid; else hi = mid;
+                }
+                sOwn[j] = (uint16_t)lo;
+            }
+            __syncthreads();
+            // one in-edge per lane, K lanes per virtual node, TwoSum tree per virtual node
+#pragma unroll 2
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int slot = r * VPR + grp;
+                const uint32_t t = t0 + slot;
+                float c = 0.0f;
+                if (t < t1) {
+                    const int k = sOwn[slot];
+                    const uint32_t eoff = (t - sS[k]) * K + lj;
+                    if (eoff < sDeg[k]) c = a.c_old[a.col[sRow[k] + eoff]];
+                }
+                Comp g = group_reduce<K>(Comp{c, 0.0f});
+                if (lj == 0 && t < t1) sPart[slot] = make_float2(g.s, g.e);
+            }
+            __syncthreads();
+            // per entry: add its virtual nodes of this tile in order
+            for (int k = threadIdx.x; k < nent; k += WG) {
+                const uint32_t s0 = max(sS[k], t0), s1 = min(sS[k + 1], t1);
+                Comp acc{0.0f, 0.0f};
+                for (uint32_t t = s0; t < s1; ++t) {
+                    const float2 p = sPart[t - t0];
+                    acc = comp_add(acc, Comp{p.x, p.y});
+                }
+                const bool started_before = sS[k] < t0;
+                const bool continues = sS[k + 1] > t1;
+                if (!started_before && !continues) {
+                    finish_vertex(a, a.vlist[e_start + k], acc, res);
+                } else if (started_before) {
+                    a.spill[2 * tile] = make_float2(acc.s, acc.e);      // head
+                } else {
+                    a.spill[2 * tile + 1] = make_float2(acc.s, acc.e);  // tail
+                }
+            }
+            __syncthreads();
+            e_start += (uint32_t)(nnext - 1);
+        }
+    }
+    // fixed-order block reduction of the FP64 residual
+    res = wave_sum(res);
+    if (lane_id() == 0) sRed[threadIdx.x / WAVE] = res;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < WG / WAVE; ++i) t += sRed[i];
+        a.res_part[blockIdx.x] = t;
+    }
+}
+
+// Vertices whose virtual nodes span tiles: tail of the first tile + heads of the following
+// tiles, in tile order.  `span` lists their vlist positions.
+__global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __restrict__ span,
+                                               uint32_t nspan, uint32_t TV, double* res_part) {
+    __shared__ double sRed[WG / WAVE];
+    if (a.ctl->done) return;
+    double res = 0.0;
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < nspan; i += gridDim.x * WG) {
+        const uint32_t pos = span[i];
+        const uint32_t s0 = a.vscan[pos], s1 = a.vscan[pos + 1];
+        const uint32_t ta = s0 / TV, tb = (s1 - 1) / TV;
+        const float2 tl = a.spill[2 * ta + 1];
+        Comp acc{tl.x, tl.y};
+        for (uint32_t t = ta + 1; t <= tb; ++t) {
+            const float2 h = a.spill[2 * t];
+            acc = comp_add(acc, Comp{h.x, h.y});
+        }
+        finish_vertex(a, a.vlist[pos], acc, res);
+    }
+    res = wave_sum(res);
+    if (lane_id() == 0) sRed[threadIdx.x / WAVE] = res;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < WG / WAVE; ++i) t += sRed[i];
+        res_part[blockIdx.x] = t;
+    }
+}
+
+// Vertices without in-edges hold (1-d)/N from the first iteration on.
+__global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint64_t* __restrict__ out_row,
+                              uint32_t n, const float* __restrict__ x_old, const float* __restrict__ xl_old,
+                              float* __restrict__ x_new, float* __restrict__ xl_new,
+                              float* __restrict__ c_new, float base, double* res_part, const Ctl* ctl) {
+    __shared__ double sRed[WG / WAVE];
+    if (ctl->done) return;
+    double res = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        if (in_row[v + 1] != in_row[v]) continue;
+        x_new[v] = base;
+        xl_new[v] = 0.0f;
+        const uint64_t d = out_row[v + 1] - out_row[v];
+        c_new[v] = d ? base / (float)d : 0.0f;
+        res += fabs((double)base - ((double)x_old[v] + (double)xl_old[v]));
+    }
+    res = wave_sum(res);
+    if (lane_id() == 0) sRed[threadIdx.x / WAVE] = res;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < WG / WAVE; ++i) t += sRed[i];
+        res_part[blockIdx.x] = t;
+    }
+}
+
+// Sums the per-workgroup residuals in a fixed order and decides convergence.
+__global__ void k_pr_reduce(const double* __restrict__ part, uint32_t nparts, Ctl* ctl, uint32_t it,
+                            double eps) {
+    __shared__ double sRed[WG];
+    if (ctl->done) return;
+    double t = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += WG) t += part[i];
+    sRed[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = WG / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sRed[threadIdx.x] += sRed[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctl->residual[it & 3] = sRed[0];
+        ctl->iters = it;
+        if (sRed[0] < eps) ctl->done = 1;
+    }
+}
+
+__global__ void k_pr_init(const uint64_t* __restrict__ out_row, uint32_t n, float x0, float* __restrict__ x,
+                          float* __restrict__ xl, float* __restrict__ c) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        x[v] = x0;
+        xl[v] = 0.0f;
+        const uint64_t d = out_row[v + 1] - out_row[v];
+        c[v] = d ? x0 / (float)d : 0.0f;
+    }
+}
+
+__global__ void k_pr_output(const float* __restrict__ hi, const float* __restrict__ lo, uint32_t n,
+                            float* __restrict__ out) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = hi[v] + lo[v];
+}
+
+__global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV,
+                                uint32_t* __restrict__ span, uint32_t* __restrict__ count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nz;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s0 = vscan[i], s1 = vscan[i + 1];
+        if (s0 / TV != (s1 - 1) / TV) span[atomicAdd(count, 1u)] = (uint32_t)i;
+    }
+}
+
+}  // namespace
+
+eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                        eeg_stats* st) {
+    const auto t_host0 = std::chrono::steady_clock::now();
+    const bool personalised = (algo == EEG_PPR);
+    const bool push = personalised || o.variant == EEG_ASYNC_PUSH_TD || o.variant == EEG_ASYNC_PUSH_DD;
+    if (push) return fail(EEG_ERR_UNSUPPORTED, "residual-push PageRank not built yet");
+    const uint32_t K = o.K ? o.K : 8;
+    if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
+    const float d = o.damping > 0 ? o.damping : 0.85f;
+    if (!(d > 0 && d < 1)) return fail(EEG_ERR_INVALID_ARG, "damping must be in (0,1)");
+    const double eps = o.epsilon > 0 ? o.epsilon : 1e-7;
+    const uint32_t max_iters = o.max_iters ? o.max_iters : 10000;
+    const uint32_t n = g->n;
+    hipStream_t s = g->stream;
+    Work& w = g->work;
+    EEG_TRY(ensure_in(g));
+    EEG_TRY(ensure_tigr(g, g->in, g->tin, K));
+    const Tigr& t = g->tin[k_index(K)];
+    const uint32_t TV = TILE_EDGES / K;
+    const uint64_t ntiles = (t.P + TV - 1) / TV;
+
+    EEG_TRY(w.ctl.alloc(sizeof(Ctl)));
+    if (!w.host_ctl) EEG_HIP(hipHostMalloc((void**)&w.host_ctl, 2 * sizeof(Ctl), hipHostMallocDefault));
+    for (int i = 0; i < 2; ++i) {
+        EEG_TRY(w.fx[i].alloc((size_t)n * 4 + 4));
+        EEG_TRY(w.fc[i].alloc((size_t)n * 4 + 4));
+        EEG_TRY(w.fl[i].alloc((size_t)n * 4 + 4));
+    }
+    EEG_TRY(w.spill.alloc((size_t)(ntiles + 1) * 2 * sizeof(float2)));
+    // vertices whose virtual nodes cross a tile boundary (finished by k_pr_fix)
+    DevMem span, cnt;
+    EEG_TRY(span.alloc((size_t)ntiles * 4 + 4));
+    EEG_TRY(cnt.alloc(4));
+    EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    if (t.nz) {
+        k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
+            t.vscan.as<uint32_t>(), t.nz, TV, span.as<uint32_t>(), cnt.as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+    }
+    uint32_t nspan = 0;
+    EEG_HIP(hipMemcpyAsync(&nspan, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    EEG_HIP(hipStreamSynchronize(s));
+    // the fix-up order only matters within a vertex, so the list order is irrelevant
+
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
+    const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * 8));
+    const unsigned grid_fix = std::max(1u, std::min(1024u, (nspan + WG - 1) / WG));
+    const unsigned grid_iso = std::max(1u, std::min(4096u, (n + WG - 1) / WG));
+    const uint32_t nparts = grid_pull + grid_fix + grid_iso;
+    EEG_TRY(w.red.alloc((size_t)nparts * sizeof(double)));
+    EEG_HIP(hipMemsetAsync(w.red.p, 0, (size_t)nparts * sizeof(double), s));
+
+    Ctl* ctl = w.ctl.as<Ctl>();
+    EEG_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
+    const float x0 = 1.0f / (float)n;
+    const float base = (1.0f - d) / (float)n;
+    k_pr_init<<<grid_iso, WG, 0, s>>>(g->out.rowp(), n, x0, w.fx[0].as<float>(), w.fl[0].as<float>(),
+                                      w.fc[0].as<float>());
+    EEG_HIP(hipGetLastError());
+
+    PullArgs pa{};
+    pa.row = g->in.rowp();
+    pa.col = g->in.colp();
+    pa.vlist = t.vlist.as<uint32_t>();
+    pa.vscan = t.vscan.as<uint32_t>();
+    pa.nz = t.nz;
+    pa.P = (uint32_t)t.P;
+    pa.out_row = g->out.rowp();
+    pa.spill = w.spill.as<float2>();
+    pa.ctl = ctl;
+    pa.base = base;
+    pa.damping = d;
+    double* red = w.red.as<double>();
+    auto pull_fn = (K == 4) ? k_pr_pull<4> : (K == 8) ? k_pr_pull<8> : (K == 16) ? k_pr_pull<16> : k_pr_pull<32>;
+
+    hipEvent_t ev_start;
+    EEG_TRY(w.event(0, &ev_start));
+    auto launch = [&](uint32_t it) -> eeg_status {
+        pa.it = it;
+        pa.x_old = w.fx[(it - 1) & 1].as<float>();
+        pa.xl_old = w.fl[(it - 1) & 1].as<float>();
+        pa.c_old = w.fc[(it - 1) & 1].as<float>();
+        pa.x_new = w.fx[it & 1].as<float>();
+        pa.xl_new = w.fl[it & 1].as<float>();
+        pa.c_new = w.fc[it & 1].as<float>();
+        pa.res_part = red;
+        hipEvent_t eb, ea;
+        if (o.timing) {
+            EEG_TRY(w.event(2 * (size_t)it - 1, &eb));
+            EEG_HIP(hipEventRecord(eb, s));
+        }
+        if (t.P) {
+            void* args[] = {&pa};
+            EEG_HIP(hipLaunchKernel((const void*)pull_fn, dim3(grid_pull), dim3(WG), args, 0, s));
+        } else {
+            EEG_HIP(hipMemsetAsync(red, 0, grid_pull * sizeof(double), s));
+        }
+        if (nspan) {
+            k_pr_fix<<<grid_fix, WG, 0, s>>>(pa, span.as<uint32_t>(), nspan, TV, red + grid_pull);
+            EEG_HIP(hipGetLastError());
+        }
+        if (it <= 2) {
+            k_pr_isolated<<<grid_iso, WG, 0, s>>>(g->in.rowp(), g->out.rowp(), n, pa.x_old, pa.xl_old,
+                                                  pa.x_new, pa.xl_new, pa.c_new, base,
+                                                  red + grid_pull + grid_fix, ctl);
+            EEG_HIP(hipGetLastError());
+        } else if (it == 3) {
+            EEG_HIP(hipMemsetAsync(red + grid_pull + grid_fix, 0, grid_iso * sizeof(double), s));
+        }
+        k_pr_reduce<<<1, WG, 0, s>>>(red, nparts, ctl, it, eps);
+        EEG_HIP(hipGetLastError());
+        hipEvent_t ea2;
+        EEG_TRY(w.event(2 * (size_t)it, &ea2));
+        EEG_HIP(hipEventRecord(ea2, s));
+        (void)ea;
+        return EEG_OK;
+    };
+    if (!nspan) EEG_HIP(hipMemsetAsync(red + grid_pull, 0, grid_fix * sizeof(double), s));
+
+    EEG_HIP(hipEventRecord(ev_start, s));
+    uint32_t it = 1;
+    eeg_status rc = EEG_OK;
+    {
+        uint32_t chunk = 4;
+        hipEvent_t poll[2];
+        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
+        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        auto submit = [&](int slot) -> eeg_status {
+            for (uint32_t k = 0; k < chunk; ++k, ++it) EEG_TRY(launch(it));
+            EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            EEG_HIP(hipEventRecord(poll[slot], s));
+            chunk = std::min<uint32_t>(chunk * 2, 16);
+            return EEG_OK;
+        };
+        int cur = 0;
+        rc = submit(0);
+        while (rc == EEG_OK) {
+            if (it <= max_iters) rc = submit(cur ^ 1);
+            if (rc != EEG_OK) break;
+            EEG_HIP(hipEventSynchronize(poll[cur]));
+            const Ctl& h = w.host_ctl[cur];
+            if (h.done) break;
+            if (it > max_iters) {
+                EEG_HIP(hipEventSynchronize(poll[cur ^ 1]));
+                if (!w.host_ctl[cur ^ 1].done)
+                    rc = fail(EEG_ERR_NOT_CONVERGED, "PageRank residual above %g after %u iterations", eps, max_iters);
+                break;
+            }
+            cur ^= 1;
+        }
+        (void)hipEventDestroy(poll[0]);
+        (void)hipEventDestroy(poll[1]);
+    }
+    EEG_HIP(hipStreamSynchronize(s));
+    EEG_HIP(hipMemcpy(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const Ctl hc = w.host_ctl[0];
+    const uint32_t iters = hc.iters;
+    // result: the buffer written by the last iteration, copied into the label slot
+    EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
+    k_pr_output<<<grid_iso, WG, 0, s>>>(w.fx[iters & 1].as<float>(), w.fl[iters & 1].as<float>(), n,
+                                        w.lab.as<float>());
+    EEG_HIP(hipGetLastError());
+    EEG_HIP(hipStreamSynchronize(s));
+    if (st) {
+        hipEvent_t ev_end;
+        EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(iters, 1), &ev_end));
+        float ms = 0;
+        EEG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end));
+        st->iterations = iters;
+        st->launches = it - 1;
+        st->ms_traversal = ms;
+        st->pr_residual = hc.residual[iters & 3];
+        st->edges_relaxed = (uint64_t)iters * g->m;
+        st->vertices_touched = (uint64_t)iters * n;
+        // per iteration: 4 B source index + 4 B contribution gather per in-edge; per vertex
+        // 16 B in-row, 16 B out-row, 4 B old rank, 8 B new rank + contribution
+        st->bytes_alg = (double)iters * (8.0 * (double)g->m + 44.0 * (double)n);
+        double ksum = 0;
+        uint32_t kn = 0;
+        if (o.timing) {
+            for (uint32_t i = 1; i <= iters; ++i) {
+                hipEvent_t eb, ea;
+                EEG_TRY(w.event(2 * (size_t)i - 1, &eb));
+                EEG_TRY(w.event(2 * (size_t)i, &ea));
+                float km = 0;
+                EEG_HIP(hipEventElapsedTime(&km, eb, ea));
+                ksum += km;
+                ++kn;
+            }
+        }
+        st->ms_kernel_sum = ksum;
+        st->kernels_timed = kn;
+        st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    }
+    (void)source;
+    return rc;
+}
+
 }  // namespace eeg
