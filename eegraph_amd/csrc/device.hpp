@@ -85,7 +85,7 @@ __device__ __forceinline__ uint32_t wg_search(const uint32_t* __restrict__ S, ui
 // Relaxation rules.  cand() returns the label a neighbour would get over an edge of weight w
 // from a source labelled `lab`; ok is false when the source cannot improve anything (BFS/SSSP
 // relax only from finite labels, which removes the INF+w wrap of src/sssp.cu:62).
-enum Algo : int { A_BFS = 0, A_SSSP = 1, A_SSWP = 2, A_CC = 3 };
+enum Algo : int { A_BFS = 0, A_SSSP = 1, A_SSWP = 2, A_CC = 3, A_PRPUSH = 4 };
 
 template <int ALG>
 struct Rule;
@@ -134,7 +134,7 @@ struct Ctl {
     uint32_t done;                 // set once an iteration finds no work
     uint32_t iters;                // iterations that did work
     uint32_t err;                  // nonzero: a bound was hit (never written out of bounds)
-    uint32_t pad;
+    uint32_t rounds;               // residual push: take passes that found work
     unsigned long long stat[4];    // entries staged, vnodes pushed, edges relaxed, updates
     double residual[4];            // PR: L1 residual of the last iterations (slot i&3)
     double mass;                   // PR delta push / PPR: residual mass still unpushed

@@ -50,11 +50,14 @@ struct RelaxArgs {
     uint32_t it;        // 1-based iteration of this traversal
     uint32_t stampv;    // stamp value of this iteration
     uint32_t count_stats;
+    float* rres;        // residual push: residual array; `lab` then holds per-vertex push amounts
+    float thr;          // residual push: a vertex is queued when its residual crosses thr
 };
 
 template <int ALG, int K, int MODE>
 __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
-    using R = Rule<ALG>;
+    constexpr bool PUSH = (ALG == A_PRPUSH);
+    using R = Rule<PUSH ? A_CC : ALG>;
     constexpr int VPR = WG / K;             // virtual nodes per round
     constexpr int TV = TILE_EDGES / K;      // virtual nodes per tile
     constexpr int ROUNDS = TILE_EDGES / WG;
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                 const uint32_t lab = a.lab[v];
                 uint32_t deg = (uint32_t)(r1 - r0);
                 if (MODE == M_FLAGS && a.flag[v] != a.stampv) deg = 0;
-                if (!R::usable(lab)) deg = 0;
+                if (PUSH ? (lab == 0u) : !R::usable(lab)) deg = 0;
                 sRow[k] = r0;
                 sDeg[k] = deg;
                 sLab[k] = lab;
@@ -173,15 +176,26 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                     if (act && eoff < deg) {
                         const uint64_t e = sRow[k] + eoff;
                         const uint32_t d = a.col[e];
-                        const uint32_t w = R::weighted ? a.wgt[e] : 1u;
-                        const uint32_t nl = R::cand(lab, w);
                         ++st_edge;
-                        if (R::better(nl, a.lab[d])) {
-                            const uint32_t old = R::apply(&a.lab[d], nl);
-                            if (R::better(nl, old)) {
+                        if constexpr (PUSH) {
+                            // residual push: exactly one atomic lifts r[d] across thr
+                            const float pv = __uint_as_float(lab);
+                            const float old = atomicAdd(&a.rres[d], pv);
+                            if (old <= a.thr && old + pv > a.thr) {
                                 upd = true;
                                 dv = d;
                                 ++st_upd;
+                            }
+                        } else {
+                            const uint32_t w = R::weighted ? a.wgt[e] : 1u;
+                            const uint32_t nl = R::cand(lab, w);
+                            if (R::better(nl, a.lab[d])) {
+                                const uint32_t old = R::apply(&a.lab[d], nl);
+                                if (R::better(nl, old)) {
+                                    upd = true;
+                                    dv = d;
+                                    ++st_upd;
+                                }
                             }
                         }
                     }
@@ -190,10 +204,15 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                     bool cand = false;
                     uint32_t nv = 0;
                     if (upd) {
-                        const uint64_t r0 = a.row[dv], r1 = a.row[dv + 1];
-                        if (r1 > r0 && atomicExch(&a.stamp[dv], a.stampv) != a.stampv) {
-                            cand = true;
-                            nv = (uint32_t)((r1 - r0 + K - 1) / K);
+                        if constexpr (PUSH) {
+                            cand = true;  // crossing is unique: no dedup stamp needed
+                            nv = 1;
+                        } else {
+                            const uint64_t r0 = a.row[dv], r1 = a.row[dv + 1];
+                            if (r1 > r0 && atomicExch(&a.stamp[dv], a.stampv) != a.stampv) {
+                                cand = true;
+                                nv = (uint32_t)((r1 - r0 + K - 1) / K);
+                            }
                         }
                     }
                     const uint64_t mask = __ballot(cand);
@@ -362,6 +381,42 @@ int cu_count(int device) {
 }
 
 }  // namespace
+
+// Residual-push scatter: the push queue (entries + virtual-node prefix in Ctl slot
+// `slot_in`) sends push[v] along every out-edge of v; vertices whose residual crosses thr are
+// appended to `cand_out` (count in slot_out).
+eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t K, const uint32_t* q,
+                               const uint32_t* sq, const float* push, float* r, float thr,
+                               uint32_t* cand_out, uint32_t* cand_s_out, uint32_t cap, Ctl* ctl,
+                               uint32_t launch_it, int count_stats) {
+    RelaxArgs ra{};
+    ra.row = c.rowp();
+    ra.col = c.colp();
+    ra.lab = (uint32_t*)push;
+    ra.qin = q;
+    ra.sin = sq;
+    ra.qout = cand_out;
+    ra.sout = cand_s_out;
+    ra.ctl = ctl;
+    ra.qslot_in = (int)((launch_it - 1) % 3);
+    ra.qslot_out = (int)(launch_it % 3);
+    ra.cap_out = c.n;
+    ra.it = launch_it;
+    ra.stampv = 0;
+    ra.count_stats = count_stats ? 1u : 0u;
+    ra.rres = r;
+    ra.thr = thr;
+    KernelFn fn;
+    switch (K) {
+        case 4: fn = k_relax<A_PRPUSH, 4, M_QUEUE>; break;
+        case 8: fn = k_relax<A_PRPUSH, 8, M_QUEUE>; break;
+        case 16: fn = k_relax<A_PRPUSH, 16, M_QUEUE>; break;
+        default: fn = k_relax<A_PRPUSH, 32, M_QUEUE>; break;
+    }
+    void* args[] = {&ra};
+    EEG_HIP(hipLaunchKernel((const void*)fn, dim3((unsigned)cu_count(device) * 4), dim3(WG), args, 0, s));
+    return EEG_OK;
+}
 
 // Memory the traversal state needs beyond the graph (used by admission control too).
 size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {

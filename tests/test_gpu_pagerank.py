@@ -86,3 +86,36 @@ def test_pagerank_tiny_graphs():
         x, st = g.run("pr", 0, variant="sync_push_dd", K=4)
         assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL, f"trial {trial}"
         g.close()
+
+
+@pytest.mark.parametrize("variant", ["async_push_td", "async_push_dd"])
+@pytest.mark.parametrize("K", [4, 8, 32])
+def test_pagerank_residual_push_vs_fp64(rmat14, variant, K):
+    """Async variants: residual push with float atomics (order varies run to run), compensated
+    ranks; leftover residual mass <= eps*(1-d) bounds the L1 error by eps."""
+    n, s, d, g, row, col = rmat14
+    x, st = g.run("pr", 0, variant=variant, K=K, epsilon=1e-7)
+    xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13)
+    assert st["pr_residual"] <= 1e-7 * 0.15 * 1.01
+    assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL
+    assert np.abs(x.astype(np.float64) - xf).max() < 1e-7
+
+
+def test_personalised_pagerank_vs_fp64(rmat14):
+    n, s, d, g, row, col = rmat14
+    rng = np.random.default_rng(9)
+    for src in list(rng.integers(0, n, 3)) + [0]:
+        x, st = g.run("ppr", int(src), K=8, epsilon=1e-7)
+        xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(src))
+        assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL, f"source {src}"
+
+
+def test_personalised_pagerank_isolated_source():
+    n = 100
+    s = np.array([1, 2], np.uint32)
+    d = np.array([2, 1], np.uint32)
+    g = eg.Graph.from_edges(s, d, None, n=n)
+    x, st = g.run("ppr", 5, K=4)
+    expect = np.zeros(n)
+    expect[5] = 0.15  # the restart mass stays on a vertex without out-edges
+    assert np.abs(x - expect).sum() < 1e-7
