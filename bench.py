@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the push-style traversal path on one MI355X.
+
+Workload (one "step"): RMAT scale 22, edge factor 16 (67,108,864 edges), a/b/c/d =
+0.57/0.19/0.19/0.05, seed 2, weights U[1,64], generated directly in HBM; BFS, SSSP and SSWP
+from the maximum out-degree vertex, CC (minimum-id labels on the symmetrised list) and
+PageRank (d = 0.85, L1 residual < 1e-7), each in its sync and async push variant, K = 8.
+
+value = input edges traversed by the step's queries / wall time of the step (GTEPS), with the
+graph already resident in HBM.  Per-algorithm GTEPS use the device-event traversal time.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS & joules/run per algorithm; batched queries/s + p99 vs one-at-a-time"
+ALGS = ["bfs", "sssp", "sswp", "cc", "pr"]
+VARIANT_OF = {"sync": "sync_push_dd", "async": "async_push_dd"}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--scale", type=int, default=22)
+    p.add_argument("--edge-factor", type=int, default=16)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--K", type=int, default=8)
+    p.add_argument("--variants", default="sync,async")
+    p.add_argument("--algs", default=",".join(ALGS))
+    p.add_argument("--energy-seconds", type=float, default=1.0,
+                   help="minimum loop length per job for the energy reading")
+    p.add_argument("--cpu-cap", type=float, default=12.0,
+                   help="seconds per algorithm for the CPU baseline (extrapolated past it)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-energy", action="store_true")
+    return p.parse_args()
+
+
+def traffic_entry(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), committed
+    under profiles/ as data; None when no measurement for this kernel and workload exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        e = table.get(f"{workload}:{kernel}")
+        return None if e is None else float(e["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(scale, ef, seed, K, algs, cap):
+    """The oracle's restatement of the reference CPU path (seq_cpu sweeps over the Tigr layout,
+    serial delta-push PageRank), one thread, on the same graph."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import eegraph_amd as eg
+    import oracle_py as O
+
+    n = 1 << scale
+    s, d, w = eg.gen_rmat(scale, ef, seed=seed, wmin=1, wmax=64, threads=16)
+    m = len(s)
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    lay = O.tigr_layout(n, s, d, w, K)
+    total_edges, total_t, parts, extrapolated = 0, 0.0, {}, False
+    codes = {"bfs": O.BFS, "sssp": O.SSSP, "sswp": O.SSWP, "cc": O.CC}
+    for alg in algs:
+        if alg == "pr":
+            row, col, _ = O.csr(n, s, d)
+            t = time.perf_counter()
+            x, passes, conv = O.pr_delta_push(n, row, col, 0.85, 1e-7 * 0.15 / n, cap)
+            dt = time.perf_counter() - t
+        else:
+            L = lay
+            if alg == "cc":
+                L = O.tigr_layout(n, np.concatenate([s, d]), np.concatenate([d, s]), None, K)
+            lab = O.init_labels(codes[alg], n, src)
+            t = time.perf_counter()
+            passes, conv = O.seq_sweep(codes[alg], n, L, K, lab, cap)
+            dt = time.perf_counter() - t
+            del L
+        if not conv:
+            # passes needed is unknown; the GPU's own count of sweeps is a lower bound, so the
+            # rate below is an upper bound on the CPU's throughput
+            extrapolated = True
+        parts[alg] = {"seconds": round(dt, 3), "passes": passes, "converged": conv,
+                      "gteps": m / dt / 1e9}
+        total_edges += m
+        total_t += dt
+        log(f"cpu {alg}: {passes} passes in {dt:.2f} s (converged={conv})")
+    return {"value": total_edges / total_t / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": (f"RMAT scale {scale} EF{ef} seed {seed}, K={K}: restated seq_cpu sweeps "
+                       f"({', '.join(a for a in algs if a != 'pr')}) and serial delta-push PageRank, "
+                       f"one query each, 1 thread, cap {cap:.0f} s per algorithm"
+                       + (" (capped runs: lower bound on time)" if extrapolated else "")),
+            "per_algorithm": parts, "host_cpu": platform.processor() or platform.machine(),
+            "nproc": os.cpu_count()}
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {a.gpus}, WORLD_SIZE {world}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import eegraph_amd as eg
+    from eegraph_amd import _lib as L
+
+    algs = [x for x in a.algs.split(",") if x]
+    variants = [VARIANT_OF[v] for v in a.variants.split(",") if v]
+    jobs = [(alg, var) for alg in algs for var in variants]
+    workload = f"rmat{a.scale}_ef{a.edge_factor}_seed{a.seed}"
+
+    t = time.perf_counter()
+    g = eg.Graph.rmat(a.scale, a.edge_factor, seed=a.seed, wmin=1, wmax=64)
+    info = g.info()
+    g.prepare(a.K, L.PREP_TIGR | L.PREP_SYM | L.PREP_TRANSPOSE)
+    src, m = info["argmax_out_degree"], info["m"]
+    log(f"graph n={info['n']} m={m} built+prepared in {time.perf_counter() - t:.2f} s; source {src}")
+
+    def run(alg, var, **kw):
+        return g.run(alg, src, variant=var, K=a.K, want_labels=False, **kw)[1]
+
+    # counted run: algorithmic bytes and work per job (device counters, not in the timed region)
+    counted = {}
+    for alg, var in jobs:
+        counted[(alg, var)] = run(alg, var, count_work=True)
+    log("counted run done")
+
+    for i in range(a.warmup):
+        for alg, var in jobs:
+            run(alg, var)
+    log(f"warmup {a.warmup} done")
+
+    # timed region: every relaxation launch is bracketed by HIP events on the library's stream
+    per_job = {j: [] for j in jobs}
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        for alg, var in jobs:
+            per_job[(alg, var)].append(run(alg, var, timing=True))
+        log(f"step {k + 1}/{a.steps}")
+    wall = time.perf_counter() - t0
+    edges_per_step = m * len(jobs)
+    value = edges_per_step * a.steps / wall / 1e9
+
+    per_alg = {}
+    best = None  # dominant kernel: most device time in the timed region
+    for (alg, var), sts in per_job.items():
+        ms = sorted(s["ms_traversal"] for s in sts)[len(sts) // 2]
+        ksum = sum(s["ms_kernel_sum"] for s in sts)
+        kn = sum(s["kernels_timed"] for s in sts)
+        c = counted[(alg, var)]
+        its = max(1, c["iterations"])
+        bytes_per_launch = c["bytes_alg"] / its
+        avg_launch_ms = ksum / kn if kn else float("nan")
+        key = f"{alg}/{var}"
+        per_alg[key] = {"gteps": m / ms / 1e6, "ms": round(ms, 4), "iterations": sts[0]["iterations"],
+                        "edges_relaxed": c["edges_relaxed"], "label_updates": c["label_updates"],
+                        "kernel_ms_avg": round(avg_launch_ms, 5),
+                        "alg_gbs": bytes_per_launch / avg_launch_ms / 1e6 if kn else None}
+        if alg == "pr":
+            per_alg[key]["residual"] = sts[0]["pr_residual"]
+        if kn and (best is None or ksum > best[0]):
+            best = (ksum, alg, var, bytes_per_launch, avg_launch_ms, kn)
+
+    kernel_names = {"bfs": "k_relax<BFS>", "sssp": "k_relax<SSSP>", "sswp": "k_relax<SSWP>",
+                    "cc": "k_relax<CC>", "pr": "k_pr_pull+k_pr_fix+k_pr_reduce"}
+    roofline = None
+    if best:
+        _, alg, var, bpl, avg_ms, kn = best
+        kname = kernel_names[alg] if var == "sync_push_dd" or alg != "pr" else "k_pr_take+k_relax<PUSH>"
+        achieved = bpl / avg_ms / 1e6
+        traffic = traffic_entry(f"{alg}/{var}", workload)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": f"{kname} ({alg} {var}, K={a.K})", "launches_timed": kn,
+                    "alg_bytes_per_launch": round(bpl), "avg_launch_ms": round(avg_ms, 5)}
+
+    energy = None
+    if not a.no_energy and eg.energy_read(0) is not None:
+        energy = {}
+        e0 = eg.energy_read(0)
+        time.sleep(1.0)
+        idle_w = (eg.energy_read(0) - e0) * 1e-6 / 1.0
+        energy["idle_w"] = round(idle_w, 1)
+        for alg, var in jobs:
+            runs, t = 0, time.perf_counter()
+            e0 = eg.energy_read(0)
+            while time.perf_counter() - t < a.energy_seconds or runs < 3:
+                run(alg, var)
+                runs += 1
+            dt = time.perf_counter() - t
+            dj = (eg.energy_read(0) - e0) * 1e-6
+            energy[f"{alg}/{var}"] = {"j_per_run": round(dj / runs, 5), "avg_w": round(dj / dt, 1),
+                                      "runs": runs}
+        log("energy done")
+
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(a.scale, a.edge_factor, a.seed, a.K, algs, a.cpu_cap)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 labels; PR f32 compensated pairs, f64 residual",
+        "data": "synthetic (RMAT generated in HBM, counter-based RNG)",
+        "config": {"workload": f"{workload}: {','.join(algs)} x {','.join(variants)}, K={a.K}, "
+                               f"source=argmax out-degree ({src})",
+                   "scale": a.scale, "edge_factor": a.edge_factor, "seed": a.seed, "K": a.K,
+                   "edges": m, "queries_per_step": len(jobs)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "per_algorithm": per_alg,
+        "energy": energy,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -10,10 +10,13 @@
 //                    nodes in order with compensation (Kahan/TwoSum), vertices that span tiles
 //                    are finished by a fix-up pass in tile order.  Results are deterministic.
 //                    The L1 residual is accumulated in FP64 and reduced in a fixed order.
-//   async variants - residual (delta) push: x_v += r_v (compensated), d*r_v/deg pushed to the
-//                    out-neighbours' residual with float atomics, until every residual is below
-//                    a threshold.  Personalised PageRank always runs this way from a one-hot
-//                    residual (1-d) at the source, so it only touches the source's neighbourhood.
+//   async variants - the same gather with ranks updated in place (Gauss-Seidel in tile order):
+//                    fewer sweeps, no atomics, order of reads not fixed (result within the
+//                    tolerance, not bit-reproducible).
+//   personalised   - residual (delta) push from a one-hot residual (1-d) at the source:
+//                    x_v += r_v (compensated), d*r_v/deg pushed to the out-neighbours' residual
+//                    with float atomics until every residual is below a threshold, so only the
+//                    source's neighbourhood is touched.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -451,8 +454,11 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
                         eeg_stats* st) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const bool personalised = (algo == EEG_PPR);
-    const bool push = personalised || o.variant == EEG_ASYNC_PUSH_TD || o.variant == EEG_ASYNC_PUSH_DD;
-    if (push) return run_pagerank_push(g, algo, source, o, st);
+    if (personalised) return run_pagerank_push(g, algo, source, o, st);
+    // async variants update ranks in place (Gauss-Seidel in tile order): a gather may see a
+    // neighbour's rank of this sweep or of the last one, so the sum order is not fixed, but the
+    // iteration converges in fewer sweeps and needs no atomics
+    const bool inplace = (o.variant == EEG_ASYNC_PUSH_TD || o.variant == EEG_ASYNC_PUSH_DD);
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
     const float d = o.damping > 0 ? o.damping : 0.85f;
@@ -527,12 +533,13 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     EEG_TRY(w.event(0, &ev_start));
     auto launch = [&](uint32_t it) -> eeg_status {
         pa.it = it;
-        pa.x_old = w.fx[(it - 1) & 1].as<float>();
-        pa.xl_old = w.fl[(it - 1) & 1].as<float>();
-        pa.c_old = w.fc[(it - 1) & 1].as<float>();
-        pa.x_new = w.fx[it & 1].as<float>();
-        pa.xl_new = w.fl[it & 1].as<float>();
-        pa.c_new = w.fc[it & 1].as<float>();
+        const int ob = inplace ? 0 : (int)((it - 1) & 1), nb = inplace ? 0 : (int)(it & 1);
+        pa.x_old = w.fx[ob].as<float>();
+        pa.xl_old = w.fl[ob].as<float>();
+        pa.c_old = w.fc[ob].as<float>();
+        pa.x_new = w.fx[nb].as<float>();
+        pa.xl_new = w.fl[nb].as<float>();
+        pa.c_new = w.fc[nb].as<float>();
         pa.res_part = red;
         hipEvent_t eb, ea;
         if (o.timing) {
@@ -549,12 +556,13 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             k_pr_fix<<<grid_fix, WG, 0, s>>>(pa, span.as<uint32_t>(), nspan, TV, red + grid_pull);
             EEG_HIP(hipGetLastError());
         }
-        if (it <= 2) {
+        const uint32_t iso_iters = inplace ? 1 : 2;  // vertices without in-edges settle at once
+        if (it <= iso_iters) {
             k_pr_isolated<<<grid_iso, WG, 0, s>>>(g->in.rowp(), g->out.rowp(), n, pa.x_old, pa.xl_old,
                                                   pa.x_new, pa.xl_new, pa.c_new, base,
                                                   red + grid_pull + grid_fix, ctl);
             EEG_HIP(hipGetLastError());
-        } else if (it == 3) {
+        } else if (it == iso_iters + 1) {
             EEG_HIP(hipMemsetAsync(red + grid_pull + grid_fix, 0, grid_iso * sizeof(double), s));
         }
         k_pr_reduce<<<1, WG, 0, s>>>(red, nparts, ctl, it, eps);
@@ -607,8 +615,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     const uint32_t iters = hc.iters;
     // result: the buffer written by the last iteration, copied into the label slot
     EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
-    k_pr_output<<<grid_iso, WG, 0, s>>>(w.fx[iters & 1].as<float>(), w.fl[iters & 1].as<float>(), n,
-                                        w.lab.as<float>());
+    const int fb = inplace ? 0 : (int)(iters & 1);
+    k_pr_output<<<grid_iso, WG, 0, s>>>(w.fx[fb].as<float>(), w.fl[fb].as<float>(), n, w.lab.as<float>());
     EEG_HIP(hipGetLastError());
     EEG_HIP(hipStreamSynchronize(s));
     if (st) {

@@ -61,8 +61,9 @@ typedef enum eeg_algo {
  *   SYNC_PUSH_TD  : same sweep as ASYNC_PUSH_TD (the reference differs only in thread order).
  *   SYNC_PUSH_DD  : frontier-synchronous: the current frontier is processed, improved vertices
  *                   form the next frontier (compacted queue, or per-vertex flags).
- * For PR the sync variants run a Jacobi iteration with destination-owned compensated sums;
- * the async variants run residual (delta) push. */
+ * For PR the sync variants run a Jacobi iteration with destination-owned compensated sums
+ * (deterministic); the async variants run the same gather updating ranks in place.
+ * Personalised PageRank runs residual (delta) push from the source. */
 typedef enum eeg_variant {
     EEG_ASYNC_PUSH_TD = 0,
     EEG_ASYNC_PUSH_DD = 1,

@@ -90,22 +90,24 @@ def test_pagerank_tiny_graphs():
 
 @pytest.mark.parametrize("variant", ["async_push_td", "async_push_dd"])
 @pytest.mark.parametrize("K", [4, 8, 32])
-def test_pagerank_residual_push_vs_fp64(rmat14, variant, K):
-    """Async variants: residual push with float atomics (order varies run to run), compensated
-    ranks; leftover residual mass <= eps*(1-d) bounds the L1 error by eps."""
+def test_pagerank_inplace_vs_fp64(rmat14, variant, K):
+    """Async variants: the gather updates ranks in place (Gauss-Seidel in tile order), so the
+    read order varies run to run; the answer must stay within the tolerance."""
     n, s, d, g, row, col = rmat14
     x, st = g.run("pr", 0, variant=variant, K=K, epsilon=1e-7)
     xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13)
-    assert st["pr_residual"] <= 1e-7 * 0.15 * 1.01
+    assert st["pr_residual"] < 1e-7
     assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL
     assert np.abs(x.astype(np.float64) - xf).max() < 1e-7
 
 
 def test_personalised_pagerank_vs_fp64(rmat14):
+    """Residual push: leftover residual mass <= eps*(1-d) bounds the L1 error by eps."""
     n, s, d, g, row, col = rmat14
     rng = np.random.default_rng(9)
     for src in list(rng.integers(0, n, 3)) + [0]:
         x, st = g.run("ppr", int(src), K=8, epsilon=1e-7)
+        assert st["pr_residual"] <= 1e-7 * 0.15 * 1.01
         xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(src))
         assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL, f"source {src}"
 
