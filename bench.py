@@ -50,7 +50,24 @@ def parse():
                    help="seconds per algorithm for the CPU baseline (extrapolated past it)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-energy", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a plain 512 MiB device-to-device copy kernel three times "
+                        "(scale reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+def calibrate_copy():
+    import torch
+
+    n = 512 * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 4, "bytes_written": n * 4,
+                      "launches": 3}), flush=True)
 
 
 def traffic_entry(kernel, workload):
@@ -124,6 +141,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {a.gpus}, WORLD_SIZE {world}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
 
     import eegraph_amd as eg
     from eegraph_amd import _lib as L

@@ -77,7 +77,8 @@ def save_edgelist(path, n, src, dst, w=None):
 
 
 def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timing=False,
-              count_work=False, collect_activity=False, damping=0.85, epsilon=0.0, acc=0.0):
+              count_work=False, collect_activity=False, damping=0.85, epsilon=0.0, acc=0.0,
+              wgs_per_cu=0):
     o = L.Opts()
     o.variant = L.VARIANTS[variant] if isinstance(variant, str) else int(variant)
     o.K = K
@@ -91,6 +92,7 @@ def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timi
     o.damping = damping
     o.epsilon = epsilon
     o.acc = acc
+    o.wgs_per_cu = wgs_per_cu
     return o
 
 

@@ -9,6 +9,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <vector>
+
 #include "engine.hpp"
 
 namespace eeg {
@@ -93,25 +95,32 @@ eeg_status scan_exclusive(F f, Wr wr, uint64_t n, hipStream_t st, uint64_t* d_to
 }
 
 // ---------------------------------------------------------------- CSR build
-__global__ void k_degree_hist(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                              uint64_t m, uint32_t n, uint32_t* __restrict__ deg,
-                              uint32_t* __restrict__ bad) {
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = src[e], d = dst[e];
-        if (s >= n || d >= n) {
-            atomicAdd(bad, 1u);
-            continue;
-        }
-        atomicAdd(&deg[s], 1u);
+// Counts edges with an endpoint >= n (one atomic per wavefront that saw one, so the valid case
+// has no contention).
+__global__ void k_validate(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                           uint64_t m, uint32_t n, uint32_t* __restrict__ bad) {
+    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x; e0 < m; e0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = e0 + threadIdx.x;
+        const bool b = e < m && (src[e] >= n || dst[e] >= n);
+        const uint64_t mask = __ballot(b);
+        if (mask && lane_id() == 0) atomicAdd(bad, (uint32_t)__popcll(mask));
     }
 }
 
-struct DegLoad {  // element i < n: deg[i]; element n: 0 (so prefix n is the total)
-    const uint32_t* deg;
-    uint32_t n;
-    __device__ uint64_t operator()(uint64_t i) const { return i < n ? deg[i] : 0; }
-};
+// row[v] = first position of key v in the sorted keys (lower bound), v in [0, n].
+__global__ void k_rows_from_sorted(const uint32_t* __restrict__ keys, uint64_t m, uint32_t n,
+                                   uint64_t* __restrict__ row) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        row[v] = lo;
+    }
+}
+
 struct RowStore {
     uint64_t* row;
     __device__ void operator()(uint64_t i, uint64_t v) const { row[i] = v; }
@@ -131,6 +140,66 @@ __global__ void k_gather(const uint32_t* __restrict__ perm, const uint32_t* __re
         const uint32_t p = perm[i];
         ao[i] = a[p];
         if (b) bo[i] = b[p];
+    }
+}
+
+// ---------------------------------------------------------------- degree-ordered relabeling
+// Vertices are renumbered by decreasing total degree (ties by id), so the labels and
+// contributions that gathers hit most often sit together and stay in each XCD's L2.
+// perm[old] = new, inv[new] = old.
+__global__ void k_degree_key(const uint64_t* __restrict__ orow, const uint64_t* __restrict__ irow,
+                             uint32_t n, uint32_t* __restrict__ key) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = (orow[v + 1] - orow[v]) + (irow[v + 1] - irow[v]);
+        key[v] = 0xffffffffu - (uint32_t)(d > 0xffffffffull ? 0xffffffffull : d);
+    }
+}
+__global__ void k_invert(const uint32_t* __restrict__ inv, uint32_t n, uint32_t* __restrict__ perm) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        perm[inv[i]] = (uint32_t)i;
+}
+__global__ void k_relabel(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ a,
+                          const uint32_t* __restrict__ b, uint32_t* __restrict__ ao,
+                          uint32_t* __restrict__ bo, uint64_t m) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        ao[i] = perm[a[i]];
+        bo[i] = perm[b[i]];
+    }
+}
+// out[u] = in[perm[u]]: results of the relabeled graph in original vertex order
+__global__ void k_unpermute(const uint32_t* __restrict__ perm, uint32_t n, const uint32_t* __restrict__ in,
+                            uint32_t* __restrict__ out) {
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x)
+        out[u] = in[perm[u]];
+}
+struct OrigDegLoad {  // degree of original vertex u (element n: 0)
+    const uint64_t* row;
+    const uint32_t* perm;
+    uint32_t n;
+    __device__ uint64_t operator()(uint64_t u) const {
+        if (u >= n) return 0;
+        const uint32_t v = perm[u];
+        return row[v + 1] - row[v];
+    }
+};
+// Rows of the relabeled CSR copied back in original ids and order.
+__global__ void k_orig_rows(const uint64_t* __restrict__ row, const uint32_t* __restrict__ col,
+                            const uint32_t* __restrict__ w, const uint32_t* __restrict__ perm,
+                            const uint32_t* __restrict__ inv, uint32_t n,
+                            const uint64_t* __restrict__ orow, uint32_t* __restrict__ ocol,
+                            uint32_t* __restrict__ ow) {
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = perm[u];
+        const uint64_t r0 = row[v], d = row[v + 1] - r0, o = orow[u];
+        for (uint64_t j = 0; j < d; ++j) {
+            ocol[o + j] = inv[col[r0 + j]];
+            if (w) ow[o + j] = w[r0 + j];
+        }
     }
 }
 
@@ -170,7 +239,9 @@ uint32_t key_bits(uint32_t n) {
     return b;
 }
 
-// Builds rows of `c` from (keys, vals, w) with keys < n: histogram, scan, stable sort by key.
+// Builds rows of `c` from (keys, vals, w) with keys < n: a stable sort of (key, edge index)
+// pairs gives the order (input order within a key); row offsets are lower bounds of each
+// vertex id in the sorted keys.
 eeg_status build_rows(hipStream_t st, uint32_t n, const uint32_t* keys, const uint32_t* vals,
                       const uint32_t* w, uint64_t m, Csr& c) {
     c.n = n;
@@ -178,25 +249,21 @@ eeg_status build_rows(hipStream_t st, uint32_t n, const uint32_t* keys, const ui
     EEG_TRY(c.row.alloc((size_t)(n + 1ull) * 8));
     EEG_TRY(c.col.alloc(m * 4));
     if (w) EEG_TRY(c.w.alloc(m * 4));
-    DevMem deg, bad;
-    EEG_TRY(deg.alloc((size_t)n * 4));
+    DevMem bad;
     EEG_TRY(bad.alloc(8));
-    EEG_HIP(hipMemsetAsync(deg.p, 0, (size_t)n * 4, st));
     EEG_HIP(hipMemsetAsync(bad.p, 0, 8, st));
-    k_degree_hist<<<grid_for(m), WG, 0, st>>>(keys, vals, m, n, deg.as<uint32_t>(),
-                                              bad.as<uint32_t>());
+    k_validate<<<grid_for(m), WG, 0, st>>>(keys, vals, m, n, bad.as<uint32_t>());
     EEG_HIP(hipGetLastError());
-    EEG_TRY(scan_exclusive(DegLoad{deg.as<uint32_t>(), n}, RowStore{c.row.as<uint64_t>()},
-                           (uint64_t)n + 1, st, nullptr));
     uint32_t nbad = 0;
     EEG_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
     EEG_HIP(hipStreamSynchronize(st));
     if (nbad) return fail(EEG_ERR_INVALID_ARG, "%u edges have an endpoint >= n (%u)", nbad, n);
     if (m == 0) {
+        EEG_HIP(hipMemsetAsync(c.row.p, 0, (size_t)(n + 1ull) * 8, st));
+        EEG_HIP(hipStreamSynchronize(st));
         c.built = true;
         return EEG_OK;
     }
-    // stable order within a key: sort (key, edge index) pairs, then gather
     DevMem kout, idx, perm, tmp;
     EEG_TRY(kout.alloc(m * 4));
     EEG_TRY(idx.alloc(m * 4));
@@ -212,6 +279,9 @@ eeg_status build_rows(hipStream_t st, uint32_t n, const uint32_t* keys, const ui
     EEG_HIP(rocprim::radix_sort_pairs(tmp.p, tbytes, keys, kout.as<uint32_t>(),
                                       idx.as<uint32_t>(), perm.as<uint32_t>(), (size_t)m, 0, bits,
                                       st));
+    k_rows_from_sorted<<<grid_for((uint64_t)n + 1), WG, 0, st>>>(kout.as<uint32_t>(), m, n,
+                                                                 c.row.as<uint64_t>());
+    EEG_HIP(hipGetLastError());
     k_gather<<<grid_for(m), WG, 0, st>>>(perm.as<uint32_t>(), vals, w, c.col.as<uint32_t>(),
                                          w ? c.w.as<uint32_t>() : nullptr, m);
     EEG_HIP(hipGetLastError());
@@ -243,6 +313,13 @@ struct TigrStore {
 };
 
 __global__ void k_put_u32(uint32_t* a, uint64_t i, uint32_t v) { a[i] = v; }
+
+__global__ void k_vnode_counts(const uint64_t* __restrict__ row, uint32_t n, uint32_t K,
+                               uint32_t* __restrict__ vnc) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        vnc[v] = (uint32_t)((row[v + 1] - row[v] + K - 1) / K);
+}
 
 // ---------------------------------------------------------------- expand sources
 __global__ void k_expand_src(const uint64_t* __restrict__ row, uint32_t n, uint64_t m,
@@ -309,18 +386,93 @@ __global__ void k_rmat(uint64_t key, uint64_t wkey, uint32_t scale, uint64_t ta,
 
 }  // namespace
 
+// Row offsets of the list sorted by `keys` (keys only; used for degrees in original ids).
+static eeg_status rows_from_keys(hipStream_t st, const uint32_t* keys, uint64_t m, uint32_t n,
+                                 DevMem& row) {
+    EEG_TRY(row.alloc(((size_t)n + 1) * 8));
+    if (m == 0) {
+        EEG_HIP(hipMemsetAsync(row.p, 0, ((size_t)n + 1) * 8, st));
+        return EEG_OK;
+    }
+    DevMem kout, tmp;
+    EEG_TRY(kout.alloc(m * 4));
+    size_t tbytes = 0;
+    const uint32_t bits = key_bits(n);
+    EEG_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys, kout.as<uint32_t>(), (size_t)m, 0, bits, st));
+    EEG_TRY(tmp.alloc(tbytes));
+    EEG_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys, kout.as<uint32_t>(), (size_t)m, 0, bits, st));
+    k_rows_from_sorted<<<grid_for((uint64_t)n + 1), WG, 0, st>>>(kout.as<uint32_t>(), m, n,
+                                                                 row.as<uint64_t>());
+    EEG_HIP(hipGetLastError());
+    EEG_HIP(hipStreamSynchronize(st));
+    return EEG_OK;
+}
+
 eeg_status build_csr_from_device_edges(eeg_graph* g, const uint32_t* d_src, const uint32_t* d_dst,
                                        const uint32_t* d_w, uint64_t m) {
     hipStream_t st = g->stream;
-    EEG_TRY(build_rows(st, g->n, d_src, d_dst, d_w, m, g->out));
+    const uint32_t n = g->n;
+    {
+        DevMem bad;
+        EEG_TRY(bad.alloc(8));
+        EEG_HIP(hipMemsetAsync(bad.p, 0, 8, st));
+        if (m) {
+            k_validate<<<grid_for(m), WG, 0, st>>>(d_src, d_dst, m, n, bad.as<uint32_t>());
+            EEG_HIP(hipGetLastError());
+        }
+        uint32_t nbad = 0;
+        EEG_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
+        EEG_HIP(hipStreamSynchronize(st));
+        if (nbad) return fail(EEG_ERR_INVALID_ARG, "%u edges have an endpoint >= n (%u)", nbad, n);
+    }
+    // degrees in original ids, then the degree order
+    DevMem orow, irow;
+    EEG_TRY(rows_from_keys(st, d_src, m, n, orow));
+    EEG_TRY(rows_from_keys(st, d_dst, m, n, irow));
+    EEG_TRY(g->perm.alloc((size_t)n * 4 + 4));
+    EEG_TRY(g->inv.alloc((size_t)n * 4 + 4));
+    if (n) {
+        DevMem key, kout, ids, tmp;
+        EEG_TRY(key.alloc((size_t)n * 4));
+        EEG_TRY(kout.alloc((size_t)n * 4));
+        EEG_TRY(ids.alloc((size_t)n * 4));
+        k_degree_key<<<grid_for(n), WG, 0, st>>>(orow.as<uint64_t>(), irow.as<uint64_t>(), n,
+                                                 key.as<uint32_t>());
+        k_iota<<<grid_for(n), WG, 0, st>>>(ids.as<uint32_t>(), n);
+        EEG_HIP(hipGetLastError());
+        size_t tbytes = 0;
+        EEG_HIP(rocprim::radix_sort_pairs(nullptr, tbytes, key.as<uint32_t>(), kout.as<uint32_t>(),
+                                          ids.as<uint32_t>(), g->inv.as<uint32_t>(), (size_t)n, 0, 32, st));
+        EEG_TRY(tmp.alloc(tbytes));
+        EEG_HIP(rocprim::radix_sort_pairs(tmp.p, tbytes, key.as<uint32_t>(), kout.as<uint32_t>(),
+                                          ids.as<uint32_t>(), g->inv.as<uint32_t>(), (size_t)n, 0, 32, st));
+        k_invert<<<grid_for(n), WG, 0, st>>>(g->inv.as<uint32_t>(), n, g->perm.as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+        EEG_HIP(hipStreamSynchronize(st));
+    }
+    irow.release();
+    {
+        DevMem s2, d2;
+        EEG_TRY(s2.alloc(m * 4 + 4));
+        EEG_TRY(d2.alloc(m * 4 + 4));
+        if (m) {
+            k_relabel<<<grid_for(m), WG, 0, st>>>(g->perm.as<uint32_t>(), d_src, d_dst,
+                                                  s2.as<uint32_t>(), d2.as<uint32_t>(), m);
+            EEG_HIP(hipGetLastError());
+        }
+        EEG_TRY(build_rows(st, n, s2.as<uint32_t>(), d2.as<uint32_t>(), d_w, m, g->out));
+    }
+    g->h_perm.resize(n);
+    if (n) EEG_HIP(hipMemcpy(g->h_perm.data(), g->perm.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    // statistics in original ids
     DevMem best, nnz;
     EEG_TRY(best.alloc(8));
     EEG_TRY(nnz.alloc(4));
     EEG_HIP(hipMemsetAsync(best.p, 0, 8, st));
     EEG_HIP(hipMemsetAsync(nnz.p, 0, 4, st));
-    if (g->n) {
-        k_degree_stats<<<grid_for(g->n, 4096), WG, 0, st>>>(
-            g->out.rowp(), g->n, best.as<unsigned long long>(), nnz.as<uint32_t>());
+    if (n) {
+        k_degree_stats<<<grid_for(n, 4096), WG, 0, st>>>(orow.as<uint64_t>(), n,
+                                                         best.as<unsigned long long>(), nnz.as<uint32_t>());
         EEG_HIP(hipGetLastError());
     }
     unsigned long long b = 0;
@@ -329,8 +481,39 @@ eeg_status build_csr_from_device_edges(eeg_graph* g, const uint32_t* d_src, cons
     EEG_HIP(hipMemcpyAsync(&c, nnz.p, 4, hipMemcpyDeviceToHost, st));
     EEG_HIP(hipStreamSynchronize(st));
     g->max_deg = (uint32_t)(b >> 32);
-    g->argmax_deg = g->n ? ~(uint32_t)(b & 0xffffffffull) : 0;
+    g->argmax_deg = n ? ~(uint32_t)(b & 0xffffffffull) : 0;
     g->nonzero_out = c;
+    return EEG_OK;
+}
+
+// A CSR of the relabeled graph `src` expressed in original ids and original row order.
+eeg_status orig_csr(eeg_graph* g, const Csr& src, Csr& out) {
+    hipStream_t st = g->stream;
+    const uint32_t n = g->n;
+    out.n = n;
+    out.m = src.m;
+    EEG_TRY(out.row.alloc(((size_t)n + 1) * 8));
+    EEG_TRY(out.col.alloc(src.m * 4 + 4));
+    if (src.w.p) EEG_TRY(out.w.alloc(src.m * 4 + 4));
+    EEG_TRY(scan_exclusive(OrigDegLoad{src.rowp(), g->perm.as<uint32_t>(), n},
+                           RowStore{out.row.as<uint64_t>()}, (uint64_t)n + 1, st, nullptr));
+    if (n) {
+        k_orig_rows<<<grid_for(n, 8192), WG, 0, st>>>(src.rowp(), src.colp(), src.wp(),
+                                                       g->perm.as<uint32_t>(), g->inv.as<uint32_t>(), n,
+                                                       out.rowp(), out.col.as<uint32_t>(),
+                                                       src.w.p ? out.w.as<uint32_t>() : nullptr);
+        EEG_HIP(hipGetLastError());
+    }
+    EEG_HIP(hipStreamSynchronize(st));
+    out.built = true;
+    return EEG_OK;
+}
+
+eeg_status unpermute(eeg_graph* g, const void* d_in, void* d_out, hipStream_t st) {
+    if (!g->n) return EEG_OK;
+    k_unpermute<<<grid_for(g->n, 8192), WG, 0, st>>>(g->perm.as<uint32_t>(), g->n,
+                                                      (const uint32_t*)d_in, (uint32_t*)d_out);
+    EEG_HIP(hipGetLastError());
     return EEG_OK;
 }
 
@@ -359,6 +542,11 @@ eeg_status ensure_tigr(eeg_graph* g, Csr& c, Tigr* table, uint32_t K) {
                     (unsigned long long)t.P);
     k_put_u32<<<1, 1, 0, st>>>(t.vscan.as<uint32_t>(), t.nz, (uint32_t)t.P);
     EEG_HIP(hipGetLastError());
+    EEG_TRY(t.vnc.alloc((size_t)c.n * 4 + 4));
+    if (c.n) {
+        k_vnode_counts<<<grid_for(c.n, 8192), WG, 0, st>>>(c.rowp(), c.n, K, t.vnc.as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+    }
     EEG_HIP(hipStreamSynchronize(st));
     t.built = true;
     return EEG_OK;
@@ -413,18 +601,20 @@ eeg_status export_tigr(eeg_graph* g, uint32_t K, uint64_t* node_pointer, uint32_
     hipStream_t st = g->stream;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
+    Csr oc;  // the layout is defined on original ids in input order
+    EEG_TRY(orig_csr(g, g->out, oc));
     DevMem np, el, pp, vnoff;
     if (node_pointer) EEG_TRY(np.alloc((size_t)n * 8 + 8));
     if (edge_list) EEG_TRY(el.alloc((2 * m + n) * 4 + 4));
     if (parts) {
         EEG_TRY(pp.alloc(2 * t.P * 4 + 8));
         EEG_TRY(vnoff.alloc((size_t)n * 8 + 8));
-        EEG_TRY(scan_exclusive(VnLoad{g->out.rowp(), K}, RowStore{vnoff.as<uint64_t>()}, n, st,
+        EEG_TRY(scan_exclusive(VnLoad{oc.rowp(), K}, RowStore{vnoff.as<uint64_t>()}, n, st,
                                nullptr));
     }
     if (n) {
         k_export_tigr<<<grid_for(n, 8192), WG, 0, st>>>(
-            g->out.rowp(), g->out.colp(), g->out.wp(), n, K,
+            oc.rowp(), oc.colp(), oc.wp(), n, K,
             parts ? vnoff.as<uint64_t>() : nullptr, node_pointer ? np.as<uint64_t>() : nullptr,
             edge_list ? el.as<uint32_t>() : nullptr, parts ? pp.as<uint32_t>() : nullptr);
         EEG_HIP(hipGetLastError());

@@ -200,6 +200,7 @@ extern "C" eeg_status eeg_graph_get_info(const eeg_graph* g, eeg_graph_info* inf
     info->device = (uint32_t)g->device;
     uint64_t b = 0;
     for (const Csr* c : {&g->out, &g->sym, &g->in}) b += c->row.bytes + c->col.bytes + c->w.bytes;
+    b += g->perm.bytes + g->inv.bytes;
     for (int k = 0; k < 4; ++k)
         for (const Tigr* t : {&g->tout[k], &g->tsym[k], &g->tin[k]}) b += t->vlist.bytes + t->vscan.bytes;
     info->device_bytes = b;
@@ -240,9 +241,11 @@ extern "C" eeg_status eeg_csr_export(eeg_graph* g, uint64_t* row, uint32_t* col,
     std::lock_guard<std::mutex> lk(g->mu);
     DeviceGuard dg;
     EEG_HIP(hipSetDevice(g->device));
-    if (row) EEG_HIP(hipMemcpy(row, g->out.row.p, ((size_t)g->n + 1) * 8, hipMemcpyDeviceToHost));
-    if (col && g->m) EEG_HIP(hipMemcpy(col, g->out.col.p, g->m * 4, hipMemcpyDeviceToHost));
-    if (w && g->m && g->out.w.p) EEG_HIP(hipMemcpy(w, g->out.w.p, g->m * 4, hipMemcpyDeviceToHost));
+    Csr oc;  // HBM holds the degree-ordered graph; export in original ids and order
+    EEG_TRY(orig_csr(g, g->out, oc));
+    if (row) EEG_HIP(hipMemcpy(row, oc.row.p, ((size_t)g->n + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && g->m) EEG_HIP(hipMemcpy(col, oc.col.p, g->m * 4, hipMemcpyDeviceToHost));
+    if (w && g->m && oc.w.p) EEG_HIP(hipMemcpy(w, oc.w.p, g->m * 4, hipMemcpyDeviceToHost));
     return EEG_OK;
 }
 
@@ -263,8 +266,11 @@ extern "C" eeg_status eeg_csr_export_derived(eeg_graph* g, uint32_t which, uint6
         c = &g->in;
     }
     if (m_out) *m_out = c->m;
-    if (row) EEG_HIP(hipMemcpy(row, c->row.p, ((size_t)g->n + 1) * 8, hipMemcpyDeviceToHost));
-    if (col && c->m) EEG_HIP(hipMemcpy(col, c->col.p, c->m * 4, hipMemcpyDeviceToHost));
+    if (!row && !col) return EEG_OK;
+    Csr oc;
+    EEG_TRY(orig_csr(g, *c, oc));
+    if (row) EEG_HIP(hipMemcpy(row, oc.row.p, ((size_t)g->n + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && c->m) EEG_HIP(hipMemcpy(col, oc.col.p, c->m * 4, hipMemcpyDeviceToHost));
     return EEG_OK;
 }
 
@@ -292,20 +298,23 @@ static eeg_status run_common(eeg_graph* g, uint32_t algo, uint32_t source, const
     const auto t0 = std::chrono::steady_clock::now();
     double e0 = 0, e1 = 0;
     const bool energy = eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
-    eeg_status s = (algo == EEG_PR || algo == EEG_PPR) ? run_pagerank(g, algo, source, o, st)
-                                                        : run_traversal(g, algo, source, o, st);
+    const uint32_t isrc = (sourced && g->n) ? g->h_perm[source] : 0;  // internal id
+    eeg_status s = (algo == EEG_PR || algo == EEG_PPR) ? run_pagerank(g, algo, isrc, o, st)
+                                                        : run_traversal(g, algo, isrc, o, st);
     if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
     clear_error();
     if (s != EEG_OK && s != EEG_ERR_NOT_CONVERGED) return s;
-    if (out && g->n) {
+    if (out && g->n) {  // back to original vertex order
         const size_t bytes = (size_t)g->n * 4;
         if (out_is_device) {
             hipStream_t us = ustream ? ustream : g->stream;
             if (us != g->stream) EEG_HIP(hipStreamSynchronize(g->stream));
-            EEG_HIP(hipMemcpyAsync(out, g->work.lab.p, bytes, hipMemcpyDeviceToDevice, us));
+            EEG_TRY(unpermute(g, g->work.lab.p, out, us));
             EEG_HIP(hipStreamSynchronize(us));
         } else {
-            EEG_HIP(hipMemcpyAsync(out, g->work.lab.p, bytes, hipMemcpyDeviceToHost, g->stream));
+            EEG_TRY(g->work.out.alloc(bytes + 4));
+            EEG_TRY(unpermute(g, g->work.lab.p, g->work.out.p, g->stream));
+            EEG_HIP(hipMemcpyAsync(out, g->work.out.p, bytes, hipMemcpyDeviceToHost, g->stream));
             EEG_HIP(hipStreamSynchronize(g->stream));
         }
     }

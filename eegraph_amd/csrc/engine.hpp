@@ -82,12 +82,14 @@ struct Tigr {
     uint64_t P = 0;
     DevMem vlist;  // u32[nz]
     DevMem vscan;  // u32[nz+1]
+    DevMem vnc;    // u32[n]: ceil(deg/K) per vertex (virtual nodes a frontier entry brings)
     bool built = false;
 };
 
 // Reusable per-graph traversal state (one query at a time per handle).
 struct Work {
-    DevMem lab;         // u32[n] (or f32 ranks)
+    DevMem lab;         // u32[n] (or f32 ranks), internal vertex order
+    DevMem out;         // u32[n]: result in original vertex order before a host copy
     DevMem q[2], s[2];  // frontier queues: entry vertex ids and vnode prefix
     DevMem stamp;       // u32[n] iteration stamps for queue dedup
     DevMem flag;        // u32[n] iteration stamps for flag-scan frontiers
@@ -115,6 +117,8 @@ struct eeg_graph {
     eeg::Csr out, sym, in;
     eeg::Tigr tout[4], tsym[4], tin[4];  // index log2(K) - 2
     eeg::Work work;
+    eeg::DevMem perm, inv;           // degree order: perm[original] = internal, inv = perm^-1
+    std::vector<uint32_t> h_perm;    // host copy of perm (sources map without a device read)
     uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
     std::mutex mu;
     ~eeg_graph();
@@ -140,6 +144,8 @@ eeg_status ensure_sym(eeg_graph* g);
 eeg_status ensure_in(eeg_graph* g);
 eeg_status export_tigr(eeg_graph* g, uint32_t K, uint64_t* node_pointer, uint32_t* edge_list,
                        uint32_t* parts, uint64_t* num_parts);
+eeg_status orig_csr(eeg_graph* g, const Csr& src, Csr& out);
+eeg_status unpermute(eeg_graph* g, const void* d_in, void* d_out, hipStream_t st);
 eeg_status generate_rmat_device(uint32_t scale, uint32_t ef, double a, double b, double c,
                                 uint64_t seed, uint32_t wmin, uint32_t wmax, uint32_t* d_src,
                                 uint32_t* d_dst, uint32_t* d_w, hipStream_t stream);

@@ -41,7 +41,6 @@ __device__ __forceinline__ Comp comp_add(Comp a, Comp b) {
     const Comp t = two_sum(a.s, b.s);
     return Comp{t.s, t.e + (a.e + b.e)};
 }
-__device__ __forceinline__ float comp_value(Comp a) { return a.s + a.e; }
 
 // Butterfly over a group of K lanes: every lane ends with the same (s, e), because each step
 // combines the same two operands on both partner lanes (addition is commutative and the
@@ -164,19 +163,34 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                 sOwn[j] = (uint16_t)lo;
             }
             __syncthreads();
-            // one in-edge per lane, K lanes per virtual node, TwoSum tree per virtual node
-#pragma unroll 2
+            // one in-edge per lane, K lanes per virtual node, TwoSum tree per virtual node;
+            // the source indices of all rounds are loaded first, then all contributions, so a
+            // lane keeps ROUNDS independent requests in flight
+            uint32_t pidx[ROUNDS];
+            bool pv[ROUNDS];
+#pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
                 const int slot = r * VPR + grp;
                 const uint32_t t = t0 + slot;
-                float c = 0.0f;
+                pv[r] = false;
+                pidx[r] = 0;
                 if (t < t1) {
                     const int k = sOwn[slot];
                     const uint32_t eoff = (t - sS[k]) * K + lj;
-                    if (eoff < sDeg[k]) c = a.c_old[a.col[sRow[k] + eoff]];
+                    if (eoff < sDeg[k]) {
+                        pv[r] = true;
+                        pidx[r] = a.col[sRow[k] + eoff];
+                    }
                 }
-                Comp g = group_reduce<K>(Comp{c, 0.0f});
-                if (lj == 0 && t < t1) sPart[slot] = make_float2(g.s, g.e);
+            }
+            float pc[ROUNDS];
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) pc[r] = pv[r] ? a.c_old[pidx[r]] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int slot = r * VPR + grp;
+                const Comp g = group_reduce<K>(Comp{pc[r], 0.0f});
+                if (lj == 0 && t0 + slot < t1) sPart[slot] = make_float2(g.s, g.e);
             }
             __syncthreads();
             // per entry: add its virtual nodes of this tile in order
@@ -499,7 +513,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
 
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
-    const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * 8));
+    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : 24;
+    const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * wpc));
     const unsigned grid_fix = std::max(1u, std::min(1024u, (nspan + WG - 1) / WG));
     const unsigned grid_iso = std::max(1u, std::min(4096u, (n + WG - 1) / WG));
     const uint32_t nparts = grid_pull + grid_fix + grid_iso;

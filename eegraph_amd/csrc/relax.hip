@@ -33,6 +33,7 @@ struct RelaxArgs {
     const uint64_t* row;
     const uint32_t* col;
     const uint32_t* wgt;
+    const uint32_t* vnc;  // per vertex: number of virtual nodes, ceil(deg/K)
     uint32_t* lab;
     const uint32_t* qin;  // frontier entries (vertex ids)
     const uint32_t* sin;  // their virtual-node prefix
@@ -156,12 +157,19 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
             }
             __syncthreads();
 
-#pragma unroll 2
+            // Phase 1: each lane issues the edge loads of all its rounds before waiting on
+            // any of them; phases 2 and 3 likewise issue every label gather, then every
+            // atomic, so a tile keeps ROUNDS independent requests in flight per lane.
+            uint32_t ed[ROUNDS], ew[ROUNDS], el[ROUNDS];
+            bool ev[ROUNDS];
+#pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
                 const int slot = r * VPR + grp;
                 const uint32_t t = t0 + slot;
-                bool upd = false;
-                uint32_t dv = 0;
+                ev[r] = false;
+                ed[r] = 0;
+                ew[r] = 1u;
+                el[r] = 0;
                 if (t < t1) {
                     const int k = sOwn[slot];
                     const uint32_t eoff = (t - sS[k]) * K + lj;
@@ -175,46 +183,72 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                     if (act && lj == 0) ++st_vn;
                     if (act && eoff < deg) {
                         const uint64_t e = sRow[k] + eoff;
-                        const uint32_t d = a.col[e];
+                        ev[r] = true;
+                        ed[r] = a.col[e];
+                        if constexpr (!PUSH && R::weighted) ew[r] = a.wgt[e];
+                        el[r] = lab;
                         ++st_edge;
-                        if constexpr (PUSH) {
-                            // residual push: exactly one atomic lifts r[d] across thr
-                            const float pv = __uint_as_float(lab);
-                            const float old = atomicAdd(&a.rres[d], pv);
-                            if (old <= a.thr && old + pv > a.thr) {
-                                upd = true;
-                                dv = d;
-                                ++st_upd;
-                            }
-                        } else {
-                            const uint32_t w = R::weighted ? a.wgt[e] : 1u;
-                            const uint32_t nl = R::cand(lab, w);
-                            if (R::better(nl, a.lab[d])) {
-                                const uint32_t old = R::apply(&a.lab[d], nl);
-                                if (R::better(nl, old)) {
-                                    upd = true;
-                                    dv = d;
-                                    ++st_upd;
-                                }
-                            }
-                        }
                     }
                 }
-                if (MODE == M_QUEUE) {
-                    bool cand = false;
-                    uint32_t nv = 0;
-                    if (upd) {
-                        if constexpr (PUSH) {
-                            cand = true;  // crossing is unique: no dedup stamp needed
-                            nv = 1;
-                        } else {
-                            const uint64_t r0 = a.row[dv], r1 = a.row[dv + 1];
-                            if (r1 > r0 && atomicExch(&a.stamp[dv], a.stampv) != a.stampv) {
-                                cand = true;
-                                nv = (uint32_t)((r1 - r0 + K - 1) / K);
-                            }
-                        }
+            }
+            bool eu[ROUNDS];
+            if constexpr (PUSH) {
+                float eo[ROUNDS];
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    eo[r] = a.thr;
+                    if (ev[r]) eo[r] = atomicAdd(&a.rres[ed[r]], __uint_as_float(el[r]));
+                }
+                // residual push: exactly one atomic lifts r[d] across thr
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r)
+                    eu[r] = ev[r] && eo[r] <= a.thr && eo[r] + __uint_as_float(el[r]) > a.thr;
+            } else {
+                uint32_t ecur[ROUNDS];
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    ew[r] = R::cand(el[r], ew[r]);  // label offered to the destination
+                    ecur[r] = ev[r] ? a.lab[ed[r]] : 0u;
+                }
+                uint32_t eo[ROUNDS];
+                bool ea[ROUNDS];
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    ea[r] = ev[r] && R::better(ew[r], ecur[r]);
+                    eo[r] = ecur[r];
+                    if (ea[r]) eo[r] = R::apply(&a.lab[ed[r]], ew[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) eu[r] = ea[r] && R::better(ew[r], eo[r]);
+            }
+            // Phase 4: frontier bookkeeping.  The queue needs each improved vertex's
+            // virtual-node count and a dedup stamp; both are issued for all rounds first.
+            uint32_t env[ROUNDS];
+            if constexpr (MODE == M_QUEUE) {
+                if constexpr (PUSH) {
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? 1u : 0u;  // unique crossing
+                } else {
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? a.vnc[ed[r]] : 0u;
+                    uint32_t eso[ROUNDS];
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r) {
+                        eso[r] = a.stampv;
+                        if (env[r]) eso[r] = atomicExch(&a.stamp[ed[r]], a.stampv);
                     }
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r)
+                        if (eso[r] == a.stampv) env[r] = 0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const bool upd = eu[r];
+                const uint32_t dv = ed[r];
+                st_upd += upd;
+                if (MODE == M_QUEUE) {
+                    const bool cand = env[r] != 0;
                     const uint64_t mask = __ballot(cand);
                     if (mask) {
                         const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -224,7 +258,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                         if (cand) {
                             const uint32_t pos = base + (uint32_t)__popcll(mask & lanemask_lt());
                             sQ[pos] = dv;
-                            sQn[pos] = nv;
+                            sQn[pos] = env[r];
                         }
                     }
                 } else {
@@ -293,15 +327,16 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
 
 // ---------------------------------------------------------------- initialisation
 // Initial labels (src/eegraph.cu:38-49 BFS/SSSP, :1549-1560 SSWP, :544-553 CC): the host
-// picks the values; CC passes identity = 1 (label = own id).  Written as a select rather than
+// picks the values; CC passes the original ids, so components end labelled by their minimum
+// original id although vertices are stored in degree order.  Written as a select rather than
 // a switch on the algorithm: hipcc 7.2 lowered that switch for gfx950 with the default arm
 // unreachable (the CC label was never written).
 __global__ void k_init_labels(uint32_t* __restrict__ lab, uint32_t n, uint32_t fill,
-                              uint32_t at_source, uint32_t source, uint32_t identity) {
+                              uint32_t at_source, uint32_t source, const uint32_t* __restrict__ ids) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t x = (v == source) ? at_source : fill;
-        lab[v] = identity ? (uint32_t)v : x;
+        lab[v] = ids ? ids[v] : x;  // CC: the original id of internal vertex v
     }
 }
 
@@ -392,6 +427,7 @@ eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t
     RelaxArgs ra{};
     ra.row = c.rowp();
     ra.col = c.colp();
+    ra.vnc = nullptr;
     ra.lab = (uint32_t*)push;
     ra.qin = q;
     ra.sin = sq;
@@ -414,7 +450,7 @@ eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t
         default: fn = k_relax<A_PRPUSH, 32, M_QUEUE>; break;
     }
     void* args[] = {&ra};
-    EEG_HIP(hipLaunchKernel((const void*)fn, dim3((unsigned)cu_count(device) * 4), dim3(WG), args, 0, s));
+    EEG_HIP(hipLaunchKernel((const void*)fn, dim3((unsigned)cu_count(device) * 16), dim3(WG), args, 0, s));
     return EEG_OK;
 }
 
@@ -433,6 +469,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
     if (o.block && o.block != WG) return fail(EEG_ERR_INVALID_ARG, "block must be %d", WG);
+    if (o.wgs_per_cu > 64) return fail(EEG_ERR_INVALID_ARG, "wgs_per_cu must be <= 64");
     int mode;
     switch (o.variant) {
         case EEG_ASYNC_PUSH_TD:
@@ -483,7 +520,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         const uint32_t fill = (alg == A_SSWP) ? 0u : INF;
         const uint32_t at_src = (alg == A_SSWP) ? INF : 0u;
         k_init_labels<<<grid_for(n, 8192), WG, 0, s>>>(w.lab.as<uint32_t>(), n, fill, at_src,
-                                                       source, alg == A_CC ? 1u : 0u);
+                                                       source, alg == A_CC ? g->inv.as<uint32_t>() : nullptr);
     }
     EEG_HIP(hipGetLastError());
     if (mode == M_ASYNC) {
@@ -508,15 +545,18 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     const int cus = cu_count(g->device);
     const uint32_t TV = TILE_EDGES / K;
     const uint64_t static_tiles = (t.P + TV - 1) / TV;
+    const uint32_t wpc_static = o.wgs_per_cu ? o.wgs_per_cu : 24;
+    const uint32_t wpc_queue = o.wgs_per_cu ? o.wgs_per_cu : 16;
     const unsigned grid_static =
-        (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * 8));
-    const unsigned grid_queue = (unsigned)cus * 4;
+        (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * wpc_static));
+    const unsigned grid_queue = (unsigned)cus * wpc_queue;
     KernelFn kfn = pick_kernel(alg, K, mode);
 
     RelaxArgs ra{};
     ra.row = c.rowp();
     ra.col = c.colp();
     ra.wgt = c.wp();
+    ra.vnc = t.vnc.as<uint32_t>();
     ra.lab = w.lab.as<uint32_t>();
     ra.stamp = w.stamp.as<uint32_t>();
     ra.flag = w.flag.as<uint32_t>();

@@ -89,10 +89,12 @@ typedef struct eeg_opts {
     uint32_t timing;           /* time every relaxation launch with HIP events */
     uint32_t count_work;       /* count entries/edges/updates on the device (adds atomics) */
     float damping;             /* PR/PPR damping (0 = 0.85) */
-    float acc;                 /* PR async: per-vertex residual push threshold (0 = auto) */
+    float acc;                 /* PPR: per-vertex residual push threshold (0 = from epsilon) */
     double epsilon;            /* PR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7);
                                   PPR: stop when the residual mass < epsilon (0 = 1e-7) */
-    uint32_t reserved[8];
+    uint32_t wgs_per_cu;       /* resident workgroups per compute unit of the tile kernels
+                                  (0 = default: 24 for sweeps, 16 for frontier queues) */
+    uint32_t reserved[7];
 } eeg_opts;
 
 typedef struct eeg_stats {
