@@ -1,0 +1,698 @@
+// Batched traversals: many sources over one HBM-resident graph in one launch sequence.
+//
+//  BFS        multi-source BFS with 64-bit visit masks (bit-parallel BFS, Then et al., "The More
+//             the Merrier: Efficient Multi-Source Graph Traversal", VLDB 2015), push form:
+//             every frontier vertex ORs its new bits into each neighbour's `next` mask where the
+//             neighbour has not seen them; a finalize pass turns next & ~seen into the next
+//             frontier and writes the level of every newly seen (vertex, query) pair.
+//  SSSP/SSWP  label planes stored vertex-major (B labels of a vertex are contiguous): a
+//             wavefront takes one edge and 64 queries, so the source and destination label
+//             reads and the atomics are contiguous 256-byte accesses.  A per-vertex change mask
+//             selects the queries a vertex still has to push.
+// Frontiers are compacted vertex queues with the Tigr virtual-node prefix, as in relax.hip,
+// and the same three-slot control rotation lets launches run ahead of the host.
+// Every answer equals the single-query run: the fixpoints are unique.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace eeg {
+namespace {
+
+constexpr uint32_t MAXW = 4;  // up to 256 queries per pass (4 x 64-bit words)
+
+unsigned grid_n(uint64_t work, unsigned cap = 16384) {
+    uint64_t g = (work + WG - 1) / WG;
+    if (g == 0) g = 1;
+    return (unsigned)(g > cap ? cap : g);
+}
+
+
+// ---------------------------------------------------------------- multi-source BFS
+struct MsArgs {
+    const uint64_t* row;
+    const uint32_t* col;
+    const uint32_t* vnc;
+    const uint32_t* qin;   // frontier vertices
+    const uint32_t* sin;   // their virtual-node prefix
+    uint64_t* seen;        // n x W
+    uint64_t* visit;       // n x W: bits a frontier vertex pushes this level
+    uint64_t* next;        // n x W: bits received this level
+    uint32_t* stamp;
+    uint32_t* cand;        // vertices that received bits (finalize list)
+    uint32_t* cand_s;
+    uint32_t* qout;        // next frontier
+    uint32_t* sout;
+    uint32_t* dist;        // n x B levels, vertex-major
+    Ctl* ctl;
+    uint32_t W, B, n;
+    uint32_t launch;       // launch index (slot rotation)
+    uint32_t level;        // level being discovered by this push (1-based)
+    uint32_t stampv;
+};
+
+template <int K>
+__global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
+    constexpr int VPR = WG / K, TV = TILE_EDGES / K, ROUNDS = TILE_EDGES / WG, EMAX = TV + 2;
+    __shared__ uint32_t sS[EMAX];
+    __shared__ uint64_t sRow[EMAX];
+    __shared__ uint32_t sDeg[EMAX];
+    __shared__ uint32_t sV[EMAX];
+    __shared__ uint16_t sOwn[TV];
+    __shared__ uint32_t sQ[TILE_EDGES];
+    __shared__ uint32_t sMisc[8];
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    const unsigned long long pk = ctl->qpack[(a.launch - 1) % 3];
+    const uint32_t nE = (uint32_t)(pk & 0xffffffffull), T = (uint32_t)(pk >> 32);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (nE == 0 || T == 0) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    if (lead) {
+        ctl->qpack[(a.launch + 1) % 3] = 0;
+        ctl->iters = a.level;
+    }
+    const uint32_t ntiles = (T + TV - 1) / TV;
+    const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
+    const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+    const int grp = threadIdx.x / K, lj = threadIdx.x % K;
+    if (c0 >= c1) return;
+    uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+    for (uint32_t tile = c0; tile < c1; ++tile) {
+        const uint32_t t0 = tile * TV, t1 = min(t0 + (uint32_t)TV, T);
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint64_t i = (uint64_t)e_start + k;
+            sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
+        }
+        if (threadIdx.x == 0) {
+            sMisc[0] = 1;
+            sMisc[1] = 1;
+            sMisc[2] = 0;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint32_t cur = sS[k], nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+            if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;
+            if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;
+        }
+        __syncthreads();
+        const int nent = (int)sMisc[0], nnext = (int)sMisc[1];
+        for (int k = threadIdx.x; k < nent; k += WG) {
+            const uint32_t v = a.qin[e_start + k];
+            sV[k] = v;
+            sRow[k] = a.row[v];
+            sDeg[k] = (uint32_t)(a.row[v + 1] - a.row[v]);
+        }
+        for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
+            const uint32_t t = t0 + j;
+            int lo = 0, hi = nent;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (sS[mid] <= t) lo = mid; else hi = mid;
+            }
+            sOwn[j] = (uint16_t)lo;
+        }
+        __syncthreads();
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int slot = r * VPR + grp;
+            const uint32_t t = t0 + slot;
+            bool got = false;
+            uint32_t u = 0;
+            if (t < t1) {
+                const int k = sOwn[slot];
+                const uint32_t eoff = (t - sS[k]) * K + lj;
+                if (eoff < sDeg[k]) {
+                    u = a.col[sRow[k] + eoff];
+                    const uint32_t v = sV[k];
+                    for (uint32_t w = 0; w < a.W; ++w) {
+                        const uint64_t m = a.visit[(uint64_t)v * a.W + w];
+                        if (!m) continue;
+                        const uint64_t add = m & ~a.seen[(uint64_t)u * a.W + w];
+                        if (add) {
+                            atomicOr((unsigned long long*)&a.next[(uint64_t)u * a.W + w],
+                                     (unsigned long long)add);
+                            got = true;
+                        }
+                    }
+                }
+            }
+            bool candq = got && atomicExch(&a.stamp[u], a.stampv) != a.stampv;
+            const uint64_t mask = __ballot(candq);
+            if (mask) {
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                uint32_t base = 0;
+                if (lane_id() == leader) base = atomicAdd(&sMisc[2], (uint32_t)__popcll(mask));
+                base = __shfl(base, leader, WAVE);
+                if (candq) sQ[base + (uint32_t)__popcll(mask & lanemask_lt())] = u;
+            }
+        }
+        __syncthreads();
+        const uint32_t cnt = sMisc[2];
+        if (cnt > 0) {
+            if (threadIdx.x == 0) {
+                const unsigned long long old = atomicAdd(&ctl->qpack[a.launch % 3], (unsigned long long)cnt);
+                sMisc[3] = (uint32_t)(old & 0xffffffffull);
+            }
+            __syncthreads();
+            const uint32_t bq = sMisc[3];
+            if ((uint64_t)bq + cnt <= a.n) {
+                for (uint32_t i = threadIdx.x; i < cnt; i += WG) a.cand[bq + i] = sQ[i];
+            } else if (threadIdx.x == 0) {
+                atomicOr(&ctl->err, 4u);
+            }
+        }
+        __syncthreads();
+        e_start += (uint32_t)(nnext - 1);
+    }
+}
+
+// Finalize: new = next & ~seen; seen |= new; visit = new; next = 0; record levels; queue the
+// vertices with new bits (and out-edges) as the next frontier.
+__global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ unsigned long long sBase;
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    const uint32_t count = (uint32_t)(ctl->qpack[(a.launch - 1) % 3] & 0xffffffffull);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (count == 0) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    if (lead) ctl->qpack[(a.launch + 1) % 3] = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * WG; base < count; base += (uint64_t)gridDim.x * WG) {
+        const uint64_t i = base + threadIdx.x;
+        uint64_t mine = 0;
+        uint32_t u = 0, nv = 0;
+        if (i < count) {
+            u = a.cand[i];
+            bool any = false;
+            for (uint32_t w = 0; w < a.W; ++w) {
+                const uint64_t idx = (uint64_t)u * a.W + w;
+                const uint64_t nw = a.next[idx] & ~a.seen[idx];
+                a.next[idx] = 0;
+                a.visit[idx] = nw;
+                if (nw) {
+                    a.seen[idx] |= nw;
+                    any = true;
+                    uint64_t bits = nw;
+                    while (bits) {
+                        const int b = __ffsll((unsigned long long)bits) - 1;
+                        bits &= bits - 1;
+                        const uint32_t q = w * 64 + (uint32_t)b;
+                        if (q < a.B) a.dist[(uint64_t)u * a.B + q] = a.level;
+                    }
+                }
+            }
+            if (any) {
+                nv = a.vnc[u];
+                if (nv) mine = 1ull | ((uint64_t)nv << 32);
+            }
+        }
+        uint64_t tot;
+        const uint64_t pre = block_excl_scan<uint64_t>(mine, sScan, &tot);
+        if (threadIdx.x == 0) sBase = tot ? atomicAdd(&ctl->qpack[a.launch % 3], (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        if (mine) {
+            const uint32_t pos = (uint32_t)(sBase & 0xffffffffull) + (uint32_t)(pre & 0xffffffffull);
+            if (pos < a.n) {
+                a.qout[pos] = u;
+                a.sout[pos] = (uint32_t)(sBase >> 32) + (uint32_t)(pre >> 32);
+            } else {
+                atomicOr(&ctl->err, 8u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- batched SSSP / SSWP
+struct PlaneArgs {
+    const uint64_t* row;
+    const uint32_t* col;
+    const uint32_t* wgt;
+    const uint32_t* vnc;
+    const uint32_t* qin;
+    const uint32_t* sin;
+    uint32_t* lab;        // n x B, vertex-major
+    uint64_t* chg;        // n x W: queries whose label improved and were not pushed yet
+    uint64_t* nchg;       // n x W: improvements of this iteration
+    uint32_t* stamp;
+    uint32_t* qout;
+    uint32_t* sout;
+    Ctl* ctl;
+    uint32_t W, B, n;
+    uint32_t it;
+    uint32_t stampv;
+    int widest;           // 0: SSSP (min, +w), 1: SSWP (max, min(.,w))
+};
+
+// One wavefront per edge and 64-query chunk.  Each wavefront walks the edges of the tile's
+// virtual nodes; lane q of the wavefront handles query (chunk*64 + q).
+template <int K>
+__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
+    constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE;
+    __shared__ uint32_t sS[EMAX];
+    __shared__ uint64_t sRow[EMAX];
+    __shared__ uint32_t sDeg[EMAX];
+    __shared__ uint32_t sV[EMAX];
+    __shared__ uint32_t sQ[TILE_EDGES];
+    __shared__ uint32_t sMisc[8];
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    const unsigned long long pk = ctl->qpack[(a.it - 1) % 3];
+    const uint32_t nE = (uint32_t)(pk & 0xffffffffull), T = (uint32_t)(pk >> 32);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (nE == 0 || T == 0) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    if (lead) {
+        ctl->qpack[(a.it + 1) % 3] = 0;
+        ctl->iters = a.it;
+    }
+    const uint32_t ntiles = (T + TV - 1) / TV;
+    const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
+    const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+    if (c0 >= c1) return;
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+    for (uint32_t tile = c0; tile < c1; ++tile) {
+        const uint32_t t0 = tile * TV, t1 = min(t0 + (uint32_t)TV, T);
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint64_t i = (uint64_t)e_start + k;
+            sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
+        }
+        if (threadIdx.x == 0) {
+            sMisc[0] = 1;
+            sMisc[1] = 1;
+            sMisc[2] = 0;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint32_t cur = sS[k], nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+            if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;
+            if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;
+        }
+        __syncthreads();
+        const int nent = (int)sMisc[0], nnext = (int)sMisc[1];
+        for (int k = threadIdx.x; k < nent; k += WG) {
+            const uint32_t v = a.qin[e_start + k];
+            sV[k] = v;
+            sRow[k] = a.row[v];
+            sDeg[k] = (uint32_t)(a.row[v + 1] - a.row[v]);
+        }
+        __syncthreads();
+        // virtual nodes of this tile, one wavefront at a time
+        for (uint32_t t = t0 + wv; t < t1; t += NW) {
+            int lo = 0, hi = nent;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (sS[mid] <= t) lo = mid; else hi = mid;
+            }
+            const int k = lo;
+            const uint32_t v = sV[k];
+            const uint32_t e0 = (t - sS[k]) * K, e1 = min(e0 + (uint32_t)K, sDeg[k]);
+            for (uint32_t w = 0; w < a.W; ++w) {
+                const uint64_t m = a.chg[(uint64_t)v * a.W + w];
+                if (!m) continue;
+                const uint32_t q = w * 64 + (uint32_t)lane;
+                const bool mine = (m >> lane) & 1ull;
+                const uint32_t lv = mine ? a.lab[(uint64_t)v * a.B + q] : 0u;
+                for (uint32_t e = e0; e < e1; ++e) {
+                    const uint32_t u = a.col[sRow[k] + e];
+                    const uint32_t wt = a.wgt[sRow[k] + e];
+                    bool upd = false;
+                    if (mine) {
+                        uint32_t cand;
+                        bool ok;
+                        if (a.widest) {
+                            cand = lv < wt ? lv : wt;
+                            ok = lv > 0;
+                        } else {
+                            const uint64_t s = (uint64_t)lv + wt;
+                            cand = s < INF ? (uint32_t)s : INF;
+                            ok = lv < INF;
+                        }
+                        uint32_t* p = &a.lab[(uint64_t)u * a.B + q];
+                        const uint32_t cur = *p;
+                        if (ok && (a.widest ? cand > cur : cand < cur)) {
+                            const uint32_t old = a.widest ? atomicMax(p, cand) : atomicMin(p, cand);
+                            upd = a.widest ? cand > old : cand < old;
+                        }
+                    }
+                    const uint64_t um = __ballot(upd);
+                    if (um && lane == 0) {
+                        atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
+                        // queue vertices that have out-edges (each queued entry owns >= 1 vnode)
+                        if (a.vnc[u] && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+                            const uint32_t pos = atomicAdd(&sMisc[2], 1u);
+                            if (pos < TILE_EDGES) sQ[pos] = u;
+                            else atomicOr(&ctl->err, 16u);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // queue the improved vertices with their virtual-node counts
+        const uint32_t cnt = min(sMisc[2], (uint32_t)TILE_EDGES);
+        if (cnt > 0) {
+            const uint32_t per = (cnt + WG - 1) / WG, b0 = threadIdx.x * per;
+            uint64_t loc = 0;
+            for (uint32_t i = 0; i < per; ++i)
+                if (b0 + i < cnt) loc += 1ull | ((uint64_t)a.vnc[sQ[b0 + i]] << 32);
+            uint64_t tot;
+            uint64_t pre = block_excl_scan<uint64_t>(loc, sScan, &tot);
+            if (threadIdx.x == 0) {
+                const unsigned long long old = atomicAdd(&ctl->qpack[a.it % 3], (unsigned long long)tot);
+                sMisc[3] = (uint32_t)(old & 0xffffffffull);
+                sMisc[4] = (uint32_t)(old >> 32);
+            }
+            __syncthreads();
+            uint32_t bq = sMisc[3] + (uint32_t)(pre & 0xffffffffull);
+            uint32_t bv = sMisc[4] + (uint32_t)(pre >> 32);
+            for (uint32_t i = 0; i < per; ++i) {
+                if (b0 + i >= cnt) break;
+                const uint32_t u = sQ[b0 + i], nv = a.vnc[u];
+                if (bq < a.n) {
+                    a.qout[bq] = u;
+                    a.sout[bq] = bv;
+                } else {
+                    atomicOr(&ctl->err, 32u);
+                }
+                ++bq;
+                bv += nv;
+            }
+        }
+        __syncthreads();
+        e_start += (uint32_t)(nnext - 1);
+    }
+}
+
+// Vertices of the frontier just pushed forget their change masks; vertices improved this
+// iteration take theirs from nchg.  Runs over the previous frontier, then the new one.
+__global__ void k_plane_rotate(const uint32_t* __restrict__ q, const Ctl* ctl, int slot,
+                               uint64_t* __restrict__ from, uint64_t* __restrict__ to, uint32_t W,
+                               int move) {
+    if (ctl->done) return;
+    const uint32_t count = (uint32_t)(ctl->qpack[slot] & 0xffffffffull);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)count * W;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t idx = (uint64_t)q[i / W] * W + (i % W);
+        if (move) {
+            to[idx] = from[idx];
+            from[idx] = 0;
+        } else {
+            to[idx] = 0;
+        }
+    }
+}
+
+// out[q * n + u] = planes[perm[u] * B + q]: answers in query-major, original vertex order
+__global__ void k_planes_out(const uint32_t* __restrict__ planes, const uint32_t* __restrict__ perm,
+                             uint32_t n, uint32_t B, uint32_t nq, uint32_t* __restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * nq;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = (uint32_t)(i / n), u = (uint32_t)(i % n);
+        out[i] = planes[(uint64_t)perm[u] * B + q];
+    }
+}
+
+__global__ void k_fill_u32b(uint32_t* __restrict__ a, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+// Seeds: entries (vertex, query) applied on the device; the first frontier is the distinct
+// source vertices with their virtual-node prefix, computed on the host.
+__global__ void k_seed(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ sq, uint32_t ns,
+                       uint64_t* __restrict__ m1, uint64_t* __restrict__ m2, uint32_t* __restrict__ lab,
+                       uint32_t W, uint32_t B, uint32_t val) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        const uint64_t idx = (uint64_t)sv[i] * W + sq[i] / 64;
+        const uint64_t bit = 1ull << (sq[i] % 64);
+        atomicOr((unsigned long long*)&m1[idx], (unsigned long long)bit);
+        if (m2) atomicOr((unsigned long long*)&m2[idx], (unsigned long long)bit);
+        lab[(uint64_t)sv[i] * B + sq[i]] = val;
+    }
+}
+
+__global__ void k_set_frontier(const uint32_t* __restrict__ fv, const uint32_t* __restrict__ fs, uint32_t nf,
+                               uint64_t packed, uint32_t* __restrict__ q, uint32_t* __restrict__ s, Ctl* ctl) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+        q[i] = fv[i];
+        s[i] = fs[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) ctl->qpack[k] = 0;
+        ctl->qpack[0] = packed;
+        ctl->done = 0;
+        ctl->iters = 0;
+        ctl->err = 0;
+    }
+}
+
+int cu_count_b(int device) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c <= 0) c = 256;
+    return c;
+}
+
+}  // namespace
+
+// Batched state of a graph handle (allocated on first use, grown as needed).
+struct BatchWork {
+    DevMem m1, m2, m3;      // seen / visit / next (BFS) or chg / nchg (planes)
+    DevMem planes;          // n x B labels
+    DevMem q[2], s[2], cand, stamp, seeds, ctl;
+    uint32_t epoch = 1;
+};
+
+static BatchWork& batch_work(eeg_graph* g) {
+    static std::mutex mu;
+    static std::vector<std::pair<eeg_graph*, BatchWork*>> table;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& e : table)
+        if (e.first == g) return *e.second;
+    table.emplace_back(g, new BatchWork());
+    return *table.back().second;
+}
+
+eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                     void* out, bool out_device, eeg_stats* st) {
+    const auto t_host0 = std::chrono::steady_clock::now();
+    if (nsrc == 0 || nsrc > 64 * MAXW) return fail(EEG_ERR_INVALID_ARG, "nsrc must be 1..%u per pass", 64 * MAXW);
+    if (algo != EEG_BFS && algo != EEG_SSSP && algo != EEG_SSWP)
+        return fail(EEG_ERR_UNSUPPORTED, "batched runs support BFS, SSSP and SSWP");
+    if ((algo == EEG_SSSP || algo == EEG_SSWP) && !g->weighted)
+        return fail(EEG_ERR_INVALID_ARG, "SSSP/SSWP need a weighted graph");
+    const uint32_t K = o.K ? o.K : 8;
+    if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32");
+    const uint32_t n = g->n;
+    for (uint32_t i = 0; i < nsrc; ++i)
+        if (sources[i] >= n) return fail(EEG_ERR_INVALID_ARG, "source %u out of range", sources[i]);
+    EEG_TRY(ensure_tigr(g, g->out, g->tout, K));
+    const Tigr& t = g->tout[k_index(K)];
+    hipStream_t s = g->stream;
+    BatchWork& bw = batch_work(g);
+    const uint32_t W = (nsrc + 63) / 64, B = W * 64;
+    const uint64_t nw = (uint64_t)n * W, nb = (uint64_t)n * B;
+    EEG_TRY(bw.m1.alloc(nw * 8 + 8));
+    EEG_TRY(bw.m2.alloc(nw * 8 + 8));
+    EEG_TRY(bw.m3.alloc(nw * 8 + 8));
+    EEG_TRY(bw.planes.alloc(nb * 4 + 4));
+    for (int i = 0; i < 2; ++i) {
+        EEG_TRY(bw.q[i].alloc((size_t)n * 4 + 4));
+        EEG_TRY(bw.s[i].alloc((size_t)n * 4 + 4));
+    }
+    EEG_TRY(bw.cand.alloc((size_t)n * 4 + 4));
+    if (!bw.stamp.p) {
+        EEG_TRY(bw.stamp.alloc((size_t)n * 4 + 4));
+        EEG_HIP(hipMemsetAsync(bw.stamp.p, 0, bw.stamp.bytes, s));
+    }
+    if (bw.epoch > 0xf0000000u) {
+        EEG_HIP(hipMemsetAsync(bw.stamp.p, 0, bw.stamp.bytes, s));
+        bw.epoch = 1;
+    }
+    EEG_TRY(bw.ctl.alloc(sizeof(Ctl)));
+    EEG_TRY(bw.seeds.alloc((size_t)4 * nsrc * 4 + 16));
+    Ctl* ctl = bw.ctl.as<Ctl>();
+    EEG_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
+
+    // seeds in internal ids; first frontier = distinct sources with out-edges
+    std::vector<uint32_t> sv(nsrc), sq(nsrc);
+    for (uint32_t i = 0; i < nsrc; ++i) {
+        sv[i] = g->h_perm[sources[i]];
+        sq[i] = i;
+    }
+    std::vector<uint32_t> fv(sv);
+    std::sort(fv.begin(), fv.end());
+    fv.erase(std::unique(fv.begin(), fv.end()), fv.end());
+    std::vector<uint32_t> hvnc(fv.size());
+    for (size_t i = 0; i < fv.size(); ++i)
+        EEG_HIP(hipMemcpyAsync(&hvnc[i], t.vnc.as<uint32_t>() + fv[i], 4, hipMemcpyDeviceToHost, s));
+    EEG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> fvv, fss;
+    uint32_t acc = 0;
+    for (size_t i = 0; i < fv.size(); ++i)
+        if (hvnc[i]) {
+            fvv.push_back(fv[i]);
+            fss.push_back(acc);
+            acc += hvnc[i];
+        }
+    uint32_t* dseed = bw.seeds.as<uint32_t>();
+    EEG_HIP(hipMemcpyAsync(dseed, sv.data(), nsrc * 4, hipMemcpyHostToDevice, s));
+    EEG_HIP(hipMemcpyAsync(dseed + nsrc, sq.data(), nsrc * 4, hipMemcpyHostToDevice, s));
+    if (!fvv.empty()) {
+        EEG_HIP(hipMemcpyAsync(dseed + 2 * nsrc, fvv.data(), fvv.size() * 4, hipMemcpyHostToDevice, s));
+        EEG_HIP(hipMemcpyAsync(dseed + 3 * nsrc, fss.data(), fss.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    const bool bfs = (algo == EEG_BFS);
+    const bool widest = (algo == EEG_SSWP);
+    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m1.as<uint64_t>(), nw);
+    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m2.as<uint64_t>(), nw);
+    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m3.as<uint64_t>(), nw);
+    k_fill_u32b<<<grid_n(nb), WG, 0, s>>>(bw.planes.as<uint32_t>(), nb, widest ? 0u : INF);
+    EEG_HIP(hipGetLastError());
+    // BFS: seen and visit get the source bits, level 0.  Planes: chg gets them.
+    k_seed<<<grid_n(nsrc), WG, 0, s>>>(dseed, dseed + nsrc, nsrc, bw.m1.as<uint64_t>(),
+                                       bfs ? bw.m2.as<uint64_t>() : nullptr, bw.planes.as<uint32_t>(), W, B,
+                                       widest ? INF : 0u);
+    k_set_frontier<<<grid_n(fvv.size() ? fvv.size() : 1), WG, 0, s>>>(
+        dseed + 2 * nsrc, dseed + 3 * nsrc, (uint32_t)fvv.size(),
+        ((unsigned long long)acc << 32) | (unsigned long long)fvv.size(), bw.q[0].as<uint32_t>(),
+        bw.s[0].as<uint32_t>(), ctl);
+    EEG_HIP(hipGetLastError());
+
+    const int cus = cu_count_b(g->device);
+    const unsigned grid = (unsigned)cus * (o.wgs_per_cu ? o.wgs_per_cu : 16);
+    const uint32_t base = bw.epoch;
+    hipEvent_t e0, e1;
+    EEG_HIP(hipEventCreate(&e0));
+    EEG_HIP(hipEventCreate(&e1));
+    EEG_HIP(hipEventRecord(e0, s));
+    Ctl hc{};
+    uint32_t launches = 0, level = 0;
+    const uint32_t max_iters = o.max_iters ? o.max_iters : n + 2;
+    // BFS: launches alternate push (odd j) and finalize (even j); planes: one push per
+    // iteration plus two rotation passes.
+    uint32_t j = 1;
+    while (true) {
+        const uint32_t chunk = 8;
+        for (uint32_t c = 0; c < chunk; ++c) {
+            if (bfs) {
+                MsArgs ma{};
+                ma.row = g->out.rowp();
+                ma.col = g->out.colp();
+                ma.vnc = t.vnc.as<uint32_t>();
+                ma.seen = bw.m1.as<uint64_t>();
+                ma.visit = bw.m2.as<uint64_t>();
+                ma.next = bw.m3.as<uint64_t>();
+                ma.stamp = bw.stamp.as<uint32_t>();
+                ma.cand = bw.cand.as<uint32_t>();
+                ma.dist = bw.planes.as<uint32_t>();
+                ma.ctl = ctl;
+                ma.W = W;
+                ma.B = B;
+                ma.n = n;
+                const uint32_t lvl = (j + 1) / 2;
+                ma.level = lvl;
+                ma.launch = j;
+                ma.stampv = base + lvl;
+                const int cur = (int)((lvl - 1) & 1);
+                ma.qin = bw.q[cur].as<uint32_t>();
+                ma.sin = bw.s[cur].as<uint32_t>();
+                ma.qout = bw.q[cur ^ 1].as<uint32_t>();
+                ma.sout = bw.s[cur ^ 1].as<uint32_t>();
+                if (j & 1) {
+                    void* args[] = {&ma};
+                    auto fn = (K == 4) ? k_ms_push<4> : (K == 8) ? k_ms_push<8> : (K == 16) ? k_ms_push<16> : k_ms_push<32>;
+                    EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
+                } else {
+                    k_ms_finalize<<<grid, WG, 0, s>>>(ma);
+                    EEG_HIP(hipGetLastError());
+                }
+            } else {
+                PlaneArgs pa{};
+                pa.row = g->out.rowp();
+                pa.col = g->out.colp();
+                pa.wgt = g->out.wp();
+                pa.vnc = t.vnc.as<uint32_t>();
+                pa.lab = bw.planes.as<uint32_t>();
+                pa.chg = bw.m1.as<uint64_t>();
+                pa.nchg = bw.m3.as<uint64_t>();
+                pa.stamp = bw.stamp.as<uint32_t>();
+                pa.ctl = ctl;
+                pa.W = W;
+                pa.B = B;
+                pa.n = n;
+                pa.it = j;
+                pa.stampv = base + j;
+                pa.widest = widest ? 1 : 0;
+                const int cur = (int)((j - 1) & 1);
+                pa.qin = bw.q[cur].as<uint32_t>();
+                pa.sin = bw.s[cur].as<uint32_t>();
+                pa.qout = bw.q[cur ^ 1].as<uint32_t>();
+                pa.sout = bw.s[cur ^ 1].as<uint32_t>();
+                void* args[] = {&pa};
+                auto fn = (K == 4) ? k_plane_push<4> : (K == 8) ? k_plane_push<8> : (K == 16) ? k_plane_push<16> : k_plane_push<32>;
+                EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
+                // forget the pushed frontier's masks, then move this iteration's improvements in
+                k_plane_rotate<<<2048, WG, 0, s>>>(bw.q[cur].as<uint32_t>(), ctl, (int)((j - 1) % 3),
+                                                    nullptr, bw.m1.as<uint64_t>(), W, 0);
+                k_plane_rotate<<<2048, WG, 0, s>>>(bw.q[cur ^ 1].as<uint32_t>(), ctl, (int)(j % 3),
+                                                    bw.m3.as<uint64_t>(), bw.m1.as<uint64_t>(), W, 1);
+                EEG_HIP(hipGetLastError());
+            }
+            ++j;
+            ++launches;
+        }
+        EEG_HIP(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        EEG_HIP(hipStreamSynchronize(s));
+        if (hc.err) return fail(EEG_ERR_DEVICE, "batched frontier overflow (err=%u)", hc.err);
+        if (hc.done) break;
+        level = hc.iters;
+        if (j > 2 * max_iters + 16) return fail(EEG_ERR_NOT_CONVERGED, "batched traversal did not converge");
+    }
+    EEG_HIP(hipEventRecord(e1, s));
+    bw.epoch = base + j + 2;
+    // answers: query-major, original vertex order
+    uint32_t* dst = nullptr;
+    DevMem tmp;
+    const uint64_t outn = (uint64_t)nsrc * n;
+    if (out_device) {
+        dst = (uint32_t*)out;
+    } else {
+        EEG_TRY(tmp.alloc(outn * 4 + 4));
+        dst = tmp.as<uint32_t>();
+    }
+    k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B, nsrc, dst);
+    EEG_HIP(hipGetLastError());
+    if (!out_device && out) EEG_HIP(hipMemcpyAsync(out, dst, outn * 4, hipMemcpyDeviceToHost, s));
+    EEG_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    EEG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)level;
+    if (st) {
+        st->iterations = hc.iters;
+        st->launches = launches;
+        st->ms_traversal = ms;
+        st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    }
+    return EEG_OK;
+}
+
+}  // namespace eeg

@@ -1,6 +1,7 @@
 // C-ABI entry points for device graphs and single queries.  The reference allocates, copies
 // and frees every device array inside each eegraph_<alg> call (src/eegraph.cu:61-74,
 // 262-268); here the graph handle owns its HBM tables across queries.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -331,4 +332,43 @@ extern "C" eeg_status eeg_run_device(eeg_graph* g, uint32_t algo, uint32_t sourc
                                      const eeg_opts* opts, void* d_labels_out, void* stream,
                                      eeg_stats* stats) {
     return run_common(g, algo, source, opts, d_labels_out, true, (hipStream_t)stream, stats);
+}
+
+// ---------------------------------------------------------------- batched queries
+namespace eeg {
+eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                     void* out, bool out_device, eeg_stats* st);
+}
+
+extern "C" eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc,
+                                    const eeg_opts* opts, void* labels_out, eeg_stats* stats) {
+    clear_error();
+    if (!g || !sources || nsrc == 0) return fail(EEG_ERR_INVALID_ARG, "null graph/sources or nsrc = 0");
+    eeg_opts o;
+    memset(&o, 0, sizeof(o));
+    if (opts) o = *opts;
+    eeg_stats local;
+    eeg_stats* st = stats ? stats : &local;
+    memset(st, 0, sizeof(*st));
+    std::lock_guard<std::mutex> lk(g->mu);
+    DeviceGuard dg;
+    EEG_HIP(hipSetDevice(g->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    double e0 = 0, e1 = 0;
+    const bool energy = eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
+    // passes of at most 256 sources (4 mask words per vertex)
+    const uint32_t pass = 256;
+    for (uint32_t b = 0; b < nsrc; b += pass) {
+        const uint32_t cnt = std::min(pass, nsrc - b);
+        eeg_stats ps;
+        memset(&ps, 0, sizeof(ps));
+        void* dst = labels_out ? (void*)((uint32_t*)labels_out + (uint64_t)b * g->n) : nullptr;
+        EEG_TRY(run_batch(g, algo, sources + b, cnt, o, dst, false, &ps));
+        st->iterations = std::max(st->iterations, ps.iterations);
+        st->launches += ps.launches;
+        st->ms_traversal += ps.ms_traversal;
+    }
+    if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
+    st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return EEG_OK;
 }

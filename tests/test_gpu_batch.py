@@ -1,0 +1,63 @@
+"""Batched traversals: every row of a batched answer must equal the single-query run (and the
+oracle) bit for bit — the fixpoints are unique."""
+import numpy as np
+import pytest
+
+import eegraph_amd as eg
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+ALG = {"bfs": O.BFS, "sssp": O.SSSP, "sswp": O.SSWP}
+
+
+@pytest.fixture(scope="module")
+def rmat13():
+    s, d, w = eg.gen_rmat(13, 16, seed=4, wmin=1, wmax=64)
+    n = 1 << 13
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    yield n, s, d, w, g
+    g.close()
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+@pytest.mark.parametrize("nsrc", [1, 37, 64, 200, 256, 300])
+def test_batch_equals_single_and_oracle(rmat13, alg, nsrc):
+    n, s, d, w, g = rmat13
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.integers(0, n, nsrc).astype(np.uint32)
+    srcs[0] = int(np.argmax(np.bincount(s, minlength=n)))
+    if nsrc > 3:
+        srcs[3] = srcs[2]  # duplicate sources share a frontier entry
+    out, st = g.run_batch(alg, srcs, K=8)
+    assert out.shape == (nsrc, n)
+    for q in sorted(set([0, 1, 2, 3, nsrc - 1, nsrc // 2]) & set(range(nsrc))):
+        single, _ = g.run(alg, int(srcs[q]), variant="sync_push_dd", K=8)
+        assert np.array_equal(out[q], single), (alg, q)
+        assert np.array_equal(out[q], O.reference_labels(ALG[alg], n, s, d, w, int(srcs[q]))), (alg, q)
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+def test_batch_all_rows_vs_oracle(rmat13, alg):
+    n, s, d, w, g = rmat13
+    srcs = np.arange(0, n, n // 64, dtype=np.uint32)[:64]
+    out, _ = g.run_batch(alg, srcs, K=4)
+    row, col, wo = O.csr(n, s, d, w)
+    for q, src in enumerate(srcs):
+        ref = {"bfs": lambda: O.bfs(n, row, col, int(src)),
+               "sssp": lambda: O.dijkstra(n, row, col, wo, int(src)),
+               "sswp": lambda: O.widest(n, row, col, wo, int(src))}[alg]()
+        assert np.array_equal(out[q], ref), (alg, q)
+
+
+def test_batch_sources_without_edges():
+    n = 50
+    s = np.array([1, 2, 3], np.uint32)
+    d = np.array([2, 3, 4], np.uint32)
+    w = np.array([5, 6, 7], np.uint32)
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    srcs = np.array([0, 1, 49, 1], np.uint32)
+    for alg in ALG:
+        out, _ = g.run_batch(alg, srcs, K=4)
+        for q, src in enumerate(srcs):
+            assert np.array_equal(out[q], O.brute(ALG[alg], n, s, d, w, int(src))), (alg, q)
