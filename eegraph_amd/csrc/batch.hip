@@ -309,7 +309,10 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
             sDeg[k] = (uint32_t)(a.row[v + 1] - a.row[v]);
         }
         __syncthreads();
-        // virtual nodes of this tile, one wavefront at a time
+        // virtual nodes of this tile, one wavefront at a time: lanes first fetch the virtual
+        // node's <= K edges (one per lane) and broadcast them, then for every 64-query word
+        // with pending changes each lane (one query) issues all K destination-label reads
+        // before any atomic, so K requests per lane are in flight
         for (uint32_t t = t0 + wv; t < t1; t += NW) {
             int lo = 0, hi = nent;
             while (hi - lo > 1) {
@@ -318,37 +321,47 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
             }
             const int k = lo;
             const uint32_t v = sV[k];
-            const uint32_t e0 = (t - sS[k]) * K, e1 = min(e0 + (uint32_t)K, sDeg[k]);
+            const uint32_t e0 = (t - sS[k]) * K;
+            const uint32_t ne = min((uint32_t)K, sDeg[k] - e0);
+            uint32_t my_u = 0, my_w = 0;
+            if ((uint32_t)lane < ne) {
+                my_u = a.col[sRow[k] + e0 + lane];
+                my_w = a.wgt[sRow[k] + e0 + lane];
+            }
             for (uint32_t w = 0; w < a.W; ++w) {
                 const uint64_t m = a.chg[(uint64_t)v * a.W + w];
                 if (!m) continue;
                 const uint32_t q = w * 64 + (uint32_t)lane;
                 const bool mine = (m >> lane) & 1ull;
                 const uint32_t lv = mine ? a.lab[(uint64_t)v * a.B + q] : 0u;
-                for (uint32_t e = e0; e < e1; ++e) {
-                    const uint32_t u = a.col[sRow[k] + e];
-                    const uint32_t wt = a.wgt[sRow[k] + e];
-                    bool upd = false;
-                    if (mine) {
-                        uint32_t cand;
-                        bool ok;
-                        if (a.widest) {
-                            cand = lv < wt ? lv : wt;
-                            ok = lv > 0;
-                        } else {
-                            const uint64_t s = (uint64_t)lv + wt;
-                            cand = s < INF ? (uint32_t)s : INF;
-                            ok = lv < INF;
-                        }
-                        uint32_t* p = &a.lab[(uint64_t)u * a.B + q];
-                        const uint32_t cur = *p;
-                        if (ok && (a.widest ? cand > cur : cand < cur)) {
-                            const uint32_t old = a.widest ? atomicMax(p, cand) : atomicMin(p, cand);
-                            upd = a.widest ? cand > old : cand < old;
-                        }
+                const bool ok = mine && (a.widest ? lv > 0 : lv < INF);
+                uint32_t eu[K], ecand[K], ecur[K];
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    eu[e] = __shfl(my_u, e, WAVE);
+                    const uint32_t wt = __shfl(my_w, e, WAVE);
+                    if (a.widest) ecand[e] = lv < wt ? lv : wt;
+                    else {
+                        const uint64_t sm = (uint64_t)lv + wt;
+                        ecand[e] = sm < INF ? (uint32_t)sm : INF;
                     }
-                    const uint64_t um = __ballot(upd);
+                    ecur[e] = (ok && (uint32_t)e < ne) ? a.lab[(uint64_t)eu[e] * a.B + q] : 0u;
+                }
+                bool eup[K];
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    eup[e] = false;
+                    if (ok && (uint32_t)e < ne && (a.widest ? ecand[e] > ecur[e] : ecand[e] < ecur[e])) {
+                        uint32_t* p = &a.lab[(uint64_t)eu[e] * a.B + q];
+                        const uint32_t old = a.widest ? atomicMax(p, ecand[e]) : atomicMin(p, ecand[e]);
+                        eup[e] = a.widest ? ecand[e] > old : ecand[e] < old;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    const uint64_t um = __ballot(eup[e]);
                     if (um && lane == 0) {
+                        const uint32_t u = eu[e];
                         atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
                         // queue vertices that have out-edges (each queued entry owns >= 1 vnode)
                         if (a.vnc[u] && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
@@ -668,18 +681,22 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     EEG_HIP(hipEventRecord(e1, s));
     bw.epoch = base + j + 2;
     // answers: query-major, original vertex order
-    uint32_t* dst = nullptr;
-    DevMem tmp;
     const uint64_t outn = (uint64_t)nsrc * n;
-    if (out_device) {
-        dst = (uint32_t*)out;
-    } else {
-        EEG_TRY(tmp.alloc(outn * 4 + 4));
-        dst = tmp.as<uint32_t>();
+    if (out) {
+        uint32_t* dst = nullptr;
+        DevMem tmp;
+        if (out_device) {
+            dst = (uint32_t*)out;
+        } else {
+            EEG_TRY(tmp.alloc(outn * 4 + 4));
+            dst = tmp.as<uint32_t>();
+        }
+        k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B,
+                                                        nsrc, dst);
+        EEG_HIP(hipGetLastError());
+        if (!out_device) EEG_HIP(hipMemcpyAsync(out, dst, outn * 4, hipMemcpyDeviceToHost, s));
+        EEG_HIP(hipStreamSynchronize(s));
     }
-    k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B, nsrc, dst);
-    EEG_HIP(hipGetLastError());
-    if (!out_device && out) EEG_HIP(hipMemcpyAsync(out, dst, outn * 4, hipMemcpyDeviceToHost, s));
     EEG_HIP(hipStreamSynchronize(s));
     float ms = 0;
     EEG_HIP(hipEventElapsedTime(&ms, e0, e1));
