@@ -210,7 +210,7 @@ def main():
     roofline = None
     if best:
         _, alg, var, bpl, avg_ms, kn = best
-        kname = kernel_names[alg] if var == "sync_push_dd" or alg != "pr" else "k_pr_take+k_relax<PUSH>"
+        kname = kernel_names[alg]
         achieved = bpl / avg_ms / 1e6
         traffic = traffic_entry(f"{alg}/{var}", workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

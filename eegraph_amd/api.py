@@ -214,6 +214,16 @@ class Graph:
             d["activity"] = act[: st.activity_len].copy()
         return out, d
 
+    def autotune(self, algo, sources=None, opts=None, **kw):
+        """Times short runs and returns (tuned Opts, median traversal ms)."""
+        a = L.ALGOS[algo] if isinstance(algo, str) else int(algo)
+        o = opts if opts is not None else make_opts(**kw)
+        srcs = _u32(np.asarray(sources if sources is not None else [], dtype=np.uint32))
+        ms = C.c_double(0)
+        L.check(L.lib.eeg_autotune(self._h, a, _ptr(srcs) if len(srcs) else None, len(srcs),
+                                   C.byref(o), C.byref(ms)), f"eeg_autotune({algo})")
+        return o, ms.value
+
     def run_batch(self, algo, sources, opts=None, **kw):
         a = L.ALGOS[algo] if isinstance(algo, str) else int(algo)
         o = opts if opts is not None else make_opts(**kw)

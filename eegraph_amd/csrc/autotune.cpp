@@ -1,0 +1,172 @@
+// Autotuner: picks the variant (and frontier kind), the virtual-node degree bound K and the
+// resident workgroups per CU for one graph and algorithm by timing short runs through eeg_run.
+// The reference fixes Part_Size = 8 at compile time (include/globals.hpp:30) and takes the
+// variant from --variant (src/argument_parsing.cpp:94-106); this replaces both choices.
+//
+// Search: coordinate descent, variant first at K = 8, then K, then workgroups per CU. Each
+// candidate is timed as the median device traversal time over the sample sources after one
+// untimed run. Results are cached per graph shape (log2 N, log2 average degree, log2 degree
+// skew, log2 frontier depth) so that a second graph of the same shape reuses the choice.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "eeg_internal.hpp"
+
+namespace {
+
+using Key = std::tuple<uint32_t, int, int, int, int, bool>;
+std::mutex g_cache_mu;
+std::map<Key, eeg_opts> g_cache;
+
+int ilog2(double x) { return x <= 1.0 ? 0 : (int)std::lround(std::log2(x)); }
+
+struct Timed {
+    double ms = std::numeric_limits<double>::infinity();
+    uint32_t iters = 0;
+};
+
+// Median traversal time over the sample sources; infinity if any run fails.
+Timed time_opts(eeg_graph* g, uint32_t algo, const std::vector<uint32_t>& srcs, eeg_opts o) {
+    Timed r;
+    std::vector<double> t;
+    eeg_stats st;
+    memset(&st, 0, sizeof(st));
+    if (eeg_run(g, algo, srcs[0], &o, nullptr, &st) != EEG_OK) return r;  // untimed warm run
+    for (uint32_t s : srcs) {
+        memset(&st, 0, sizeof(st));
+        if (eeg_run(g, algo, s, &o, nullptr, &st) != EEG_OK) return r;
+        t.push_back(st.ms_traversal);
+        r.iters = std::max(r.iters, st.iterations);
+    }
+    std::sort(t.begin(), t.end());
+    r.ms = t[t.size() / 2];
+    return r;
+}
+
+}  // namespace
+
+extern "C" eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sources,
+                                   uint32_t nsample, eeg_opts* best, double* best_ms) {
+    eeg::clear_error();
+    if (!g || !best) return eeg::fail(EEG_ERR_INVALID_ARG, "null graph or result pointer");
+    if (algo > EEG_PPR) return eeg::fail(EEG_ERR_INVALID_ARG, "unknown algorithm %u", algo);
+    eeg_graph_info info;
+    EEG_TRY(eeg_graph_get_info(g, &info));
+    if (info.n == 0 || info.m == 0) return eeg::fail(EEG_ERR_INVALID_ARG, "empty graph");
+    if ((algo == EEG_SSSP || algo == EEG_SSWP) && !info.weighted)
+        return eeg::fail(EEG_ERR_INVALID_ARG, "SSSP/SSWP need a weighted graph");
+
+    std::vector<uint32_t> srcs;
+    if (sample_sources && nsample) {
+        for (uint32_t i = 0; i < nsample && i < 16; ++i) {
+            if (sample_sources[i] >= info.n)
+                return eeg::fail(EEG_ERR_INVALID_ARG, "sample source %u >= n = %u", sample_sources[i], info.n);
+            srcs.push_back(sample_sources[i]);
+        }
+    } else {
+        srcs.push_back(info.argmax_out_degree);
+    }
+
+    // Caller's numeric settings carry over; the tuned fields start from the defaults.
+    eeg_opts base;
+    memset(&base, 0, sizeof(base));
+    base.damping = best->damping;
+    base.acc = best->acc;
+    base.epsilon = best->epsilon;
+    base.max_iters = best->max_iters;
+    base.variant = EEG_SYNC_PUSH_DD;
+    base.frontier = EEG_FRONTIER_COMPACT;
+    base.K = 8;
+
+    // Every layout the search touches is built before any timing.
+    const uint32_t dir = algo == EEG_CC ? EEG_PREP_SYM : algo == EEG_PR ? EEG_PREP_TRANSPOSE : 0;
+    for (uint32_t K : {4u, 8u, 16u, 32u}) EEG_TRY(eeg_graph_prepare(g, EEG_PREP_TIGR | dir, K));
+
+    Timed cur = time_opts(g, algo, srcs, base);
+    if (!std::isfinite(cur.ms)) return EEG_ERR_DEVICE;  // eeg_run recorded the reason
+
+    const double avg = (double)info.m / info.n;
+    const Key key{algo, ilog2(info.n), ilog2(avg), ilog2(info.max_out_degree / std::max(avg, 1.0)),
+                  ilog2(cur.iters), info.weighted != 0};
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) {
+            eeg_opts o = it->second;
+            o.damping = base.damping;
+            o.acc = base.acc;
+            o.epsilon = base.epsilon;
+            o.max_iters = base.max_iters;
+            const Timed t = time_opts(g, algo, srcs, o);
+            if (std::isfinite(t.ms)) {
+                *best = o;
+                if (best_ms) *best_ms = t.ms;
+                return EEG_OK;
+            }
+        }
+    }
+
+    eeg_opts win = base;
+    // A candidate replaces the incumbent only when it is at least 3% faster, so near-ties
+    // keep the earlier (default-side) choice and repeated tuning is stable.
+    auto consider = [&](const eeg_opts& o) {
+        const Timed t = time_opts(g, algo, srcs, o);
+        if (t.ms < cur.ms * 0.97) {
+            cur = t;
+            win = o;
+        }
+    };
+
+    // 1. variant and frontier kind
+    if (algo != EEG_PPR) {
+        std::vector<eeg_opts> cand;
+        eeg_opts o = base;
+        if (algo != EEG_PR) {
+            o.frontier = EEG_FRONTIER_FLAGS;
+            cand.push_back(o);
+        }
+        o = base;
+        o.variant = EEG_ASYNC_PUSH_DD;
+        cand.push_back(o);
+        // topology-driven sweeps touch every virtual node each round: only worth timing
+        // when the frontier depth is small
+        if (algo != EEG_PR && cur.iters <= 32) {
+            o = base;
+            o.variant = EEG_ASYNC_PUSH_TD;
+            cand.push_back(o);
+        }
+        for (const auto& c : cand) consider(c);
+    }
+    // 2. virtual-node degree bound
+    {
+        const eeg_opts at = win;
+        for (uint32_t K : {4u, 16u, 32u}) {
+            eeg_opts o = at;
+            o.K = K;
+            consider(o);
+        }
+    }
+    // 3. resident workgroups per CU
+    {
+        const eeg_opts at = win;
+        for (uint32_t w : {8u, 16u, 24u, 32u}) {
+            eeg_opts o = at;
+            o.wgs_per_cu = w;
+            consider(o);
+        }
+    }
+
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache[key] = win;
+    }
+    *best = win;
+    if (best_ms) *best_ms = cur.ms;
+    return EEG_OK;
+}

@@ -7,6 +7,13 @@
 
 #include "../../include/eeg_capi.h"
 
+// Returns the status of `expr` from the enclosing function unless it is EEG_OK.
+#define EEG_TRY(expr)                \
+    do {                             \
+        eeg_status s_ = (expr);      \
+        if (s_ != EEG_OK) return s_; \
+    } while (0)
+
 namespace eeg {
 
 // Records a detail message for eeg_last_error() and returns the status unchanged.

@@ -21,12 +21,6 @@
                                __FILE__, __LINE__);                                       \
     } while (0)
 
-#define EEG_TRY(expr)                      \
-    do {                                   \
-        eeg_status s_ = (expr);            \
-        if (s_ != EEG_OK) return s_;       \
-    } while (0)
-
 namespace eeg {
 
 // Owning device allocation.

@@ -222,7 +222,7 @@ eeg_status eeg_run_device(eeg_graph* g, uint32_t algo, uint32_t source, const ee
 
 /* ---------------- batched queries ----------------
  * nsrc sources in one traversal: BFS runs multi-source BFS with 64-bit visit masks
- * (ceil(nsrc/64) words per vertex); SSSP/SSWP/PPR run label planes.  labels_out is
+ * (ceil(nsrc/64) words per vertex); SSSP/SSWP run vertex-major label planes.  labels_out is
  * nsrc x n, row q is the answer for sources[q], identical to eeg_run(sources[q]). */
 eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc,
                          const eeg_opts* opts, void* labels_out, eeg_stats* stats);
@@ -233,8 +233,13 @@ eeg_status eeg_run_packed(eeg_graph* const* graphs, uint32_t ngraphs, uint32_t a
                           eeg_stats* stats);
 
 /* ---------------- autotuner ----------------
- * Picks K, block and variant (and frontier kind) for this graph and algorithm by timing
- * short runs from the given sample sources; writes the winner into best. */
+ * No counterpart in the reference, which fixes Part_Size = 8 (include/globals.hpp:30) and
+ * takes the variant from --variant (src/argument_parsing.cpp:94-106).  Picks the variant and
+ * frontier kind, K and wgs_per_cu for this graph and algorithm by timing short runs from the
+ * sample sources (NULL/0 = the max-out-degree vertex; at most 16 are used).  damping, acc,
+ * epsilon and max_iters are read from *best and kept; the other fields are overwritten with
+ * the winner.  *best_ms receives its median traversal time.  Choices are cached per graph
+ * shape (log2 of N, average degree, degree skew and frontier depth) within the process. */
 eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sources,
                         uint32_t nsample, eeg_opts* best, double* best_ms);
 
