@@ -2,6 +2,7 @@
 // and frees every device array inside each eegraph_<alg> call (src/eegraph.cu:61-74,
 // 262-268); here the graph handle owns its HBM tables across queries.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 
@@ -23,8 +24,6 @@ Work::~Work() {
     for (auto e : events) (void)hipEventDestroy(e);
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
-
-namespace {
 
 eeg_status select_device(int device) {
     int count = 0;
@@ -55,19 +54,12 @@ eeg_status new_graph(uint32_t n, uint64_t m, bool weighted, int device, eeg_grap
         delete g;
         return fail(EEG_ERR_DEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    static std::atomic<uint64_t> next_serial{1};
+    g->serial = next_serial.fetch_add(1);
     *out = g;
     return EEG_OK;
 }
 
-struct DeviceGuard {  // restores the caller's current device
-    int prev = -1;
-    DeviceGuard() { (void)hipGetDevice(&prev); }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-}  // namespace
 }  // namespace eeg
 
 eeg_graph::~eeg_graph() {

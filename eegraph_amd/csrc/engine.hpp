@@ -114,6 +114,7 @@ struct eeg_graph {
     eeg::DevMem perm, inv;           // degree order: perm[original] = internal, inv = perm^-1
     std::vector<uint32_t> h_perm;    // host copy of perm (sources map without a device read)
     uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
+    uint64_t serial = 0;             // unique per handle in this process
     std::mutex mu;
     ~eeg_graph();
 };
@@ -130,6 +131,18 @@ inline int k_index(uint32_t K) {
     return -1;
 }
 
+// Handle creation (capi.hip): checks the device and makes it current, creates the stream.
+eeg_status select_device(int device);
+eeg_status new_graph(uint32_t n, uint64_t m, bool weighted, int device, eeg_graph** out);
+
+struct DeviceGuard {  // restores the caller's current device
+    int prev = -1;
+    DeviceGuard() { (void)hipGetDevice(&prev); }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 // Graph construction (build.hip).
 eeg_status build_csr_from_device_edges(eeg_graph* g, const uint32_t* d_src, const uint32_t* d_dst,
                                        const uint32_t* d_w, uint64_t m);
@@ -145,8 +158,9 @@ eeg_status generate_rmat_device(uint32_t scale, uint32_t ef, double a, double b,
                                 uint32_t* d_dst, uint32_t* d_w, hipStream_t stream);
 
 // Traversals (relax.hip, pagerank.hip).
+// d_seeds (device, nseeds distinct internal ids) replaces `source` when given.
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                         eeg_stats* st);
+                         eeg_stats* st, const uint32_t* d_seeds = nullptr, uint32_t nseeds = 0);
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st);
 eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t K, const uint32_t* q,

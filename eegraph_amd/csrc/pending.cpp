@@ -1,9 +1,6 @@
 // Entry points whose implementation has not landed yet.
 #include "eeg_internal.hpp"
 
-extern "C" eeg_status eeg_run_packed(eeg_graph* const*, uint32_t, uint32_t, const uint32_t*, const eeg_opts*, void* const*, eeg_stats*) {
-    return eeg::fail(EEG_ERR_UNSUPPORTED, "packed graphs not built yet");
-}
 extern "C" eeg_status eeg_server_create(eeg_graph*, const eeg_server_cfg*, eeg_server**) {
     return eeg::fail(EEG_ERR_UNSUPPORTED, "server not built yet");
 }

@@ -370,6 +370,55 @@ __global__ void k_init_ctl(Ctl* ctl, const uint64_t* row, uint32_t K, int seed, 
     }
 }
 
+// Several sources at once (graphs packed side by side, one source each): labels of the seeds,
+// and for frontier queues the first frontier with its virtual-node prefix.  `seeds` are
+// distinct internal ids.
+__global__ void k_seed_labels(uint32_t* __restrict__ lab, const uint32_t* __restrict__ seeds,
+                              uint32_t nseeds, uint32_t val) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nseeds;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        lab[seeds[i]] = val;
+}
+
+// One workgroup: appends the seeds with out-edges to q0/s0 in seed order.
+__global__ __launch_bounds__(WG) void k_seed_queue(Ctl* ctl, const uint64_t* __restrict__ row, uint32_t K,
+                                                   const uint32_t* __restrict__ seeds, uint32_t nseeds,
+                                                   uint32_t* __restrict__ q0, uint32_t* __restrict__ s0) {
+    __shared__ uint32_t sc[WG / WAVE + 1];
+    __shared__ uint64_t sv[WG / WAVE + 1];
+    uint32_t cnt = 0;
+    uint64_t vn = 0;
+    for (uint32_t b = 0; b < nseeds; b += WG) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t v = 0;
+        uint64_t d = 0;
+        if (i < nseeds) {
+            v = seeds[i];
+            d = row[v + 1] - row[v];
+        }
+        const uint32_t keep = d > 0 ? 1u : 0u;
+        const uint64_t nv = (d + K - 1) / K;
+        uint32_t tc;
+        uint64_t tv;
+        const uint32_t pc = block_excl_scan<uint32_t>(keep, sc, &tc);
+        const uint64_t pv = block_excl_scan<uint64_t>(nv, sv, &tv);
+        if (keep) {
+            q0[cnt + pc] = v;
+            s0[cnt + pc] = (uint32_t)(vn + pv);
+        }
+        cnt += tc;
+        vn += tv;
+    }
+    if (threadIdx.x == 0) ctl->qpack[0] = ((unsigned long long)vn << 32) | cnt;
+}
+
+__global__ void k_seed_flags(uint32_t* __restrict__ flag, const uint32_t* __restrict__ seeds,
+                             uint32_t nseeds, uint32_t val) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nseeds;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        flag[seeds[i]] = val;
+}
+
 unsigned grid_for(uint64_t work, unsigned cap) {
     uint64_t g = (work + WG - 1) / WG;
     if (g == 0) g = 1;
@@ -464,7 +513,7 @@ size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
 }
 
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                         eeg_stats* st) {
+                         eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
@@ -520,7 +569,10 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         const uint32_t fill = (alg == A_SSWP) ? 0u : INF;
         const uint32_t at_src = (alg == A_SSWP) ? INF : 0u;
         k_init_labels<<<grid_for(n, 8192), WG, 0, s>>>(w.lab.as<uint32_t>(), n, fill, at_src,
-                                                       source, alg == A_CC ? g->inv.as<uint32_t>() : nullptr);
+                                                       d_seeds ? 0xffffffffu : source,
+                                                       alg == A_CC ? g->inv.as<uint32_t>() : nullptr);
+        if (d_seeds && alg != A_CC && nseeds)
+            k_seed_labels<<<grid_for(nseeds, 1024), WG, 0, s>>>(w.lab.as<uint32_t>(), d_seeds, nseeds, at_src);
     }
     EEG_HIP(hipGetLastError());
     if (mode == M_ASYNC) {
@@ -531,13 +583,23 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     const uint32_t base = w.epoch;  // iteration it uses stamp base + it
     const bool seeded = (alg != A_CC);
     Ctl* ctl = w.ctl.as<Ctl>();
-    k_init_ctl<<<1, 64, 0, s>>>(ctl, c.rowp(), K, (seeded && mode == M_QUEUE) ? 1 : 0, source,
+    k_init_ctl<<<1, 64, 0, s>>>(ctl, c.rowp(), K, (seeded && mode == M_QUEUE && !d_seeds) ? 1 : 0, source,
                                 mode == M_QUEUE ? w.q[0].as<uint32_t>() : nullptr,
                                 mode == M_QUEUE ? w.s[0].as<uint32_t>() : nullptr);
     EEG_HIP(hipGetLastError());
+    if (seeded && mode == M_QUEUE && d_seeds && nseeds) {
+        k_seed_queue<<<1, WG, 0, s>>>(ctl, c.rowp(), K, d_seeds, nseeds, w.q[0].as<uint32_t>(),
+                                      w.s[0].as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+    }
     if (mode == M_FLAGS) {
-        if (seeded) k_fill_u32<<<1, 64, 0, s>>>(w.flag.as<uint32_t>() + source, 1, base + 1);
-        else k_fill_u32<<<grid_for(n, 8192), WG, 0, s>>>(w.flag.as<uint32_t>(), n, base + 1);
+        if (seeded && d_seeds) {
+            if (nseeds) k_seed_flags<<<grid_for(nseeds, 1024), WG, 0, s>>>(w.flag.as<uint32_t>(), d_seeds, nseeds, base + 1);
+        } else if (seeded) {
+            k_fill_u32<<<1, 64, 0, s>>>(w.flag.as<uint32_t>() + source, 1, base + 1);
+        } else {
+            k_fill_u32<<<grid_for(n, 8192), WG, 0, s>>>(w.flag.as<uint32_t>(), n, base + 1);
+        }
         EEG_HIP(hipGetLastError());
     }
 

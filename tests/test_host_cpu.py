@@ -154,3 +154,10 @@ def test_energy_counter_optional():
     """amd-smi may be absent or not permitted; the call then reports unsupported."""
     v = eg.energy_read(0)
     assert v is None or v >= 0
+
+
+def test_entry_points_validate_before_touching_a_device():
+    """Argument errors are reported without a GPU (no device call is made first)."""
+    assert L.lib.eeg_run_packed(None, 0, L.BFS, None, None, None, None) == L.ERR_INVALID_ARG
+    assert L.lib.eeg_autotune(None, L.BFS, None, 0, None, None) == L.ERR_INVALID_ARG
+    assert L.lib.eeg_run_batch(None, L.BFS, None, 0, None, None, None) == L.ERR_INVALID_ARG
