@@ -85,7 +85,11 @@ class EdgeList(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("label_budget_bytes", u64), ("max_batch", u32), ("K", u32),
-                ("damping", C.c_float), ("ppr_epsilon", C.c_double), ("reserved", u32 * 8)]
+                ("damping", C.c_float), ("ppr_epsilon", C.c_double), ("flags", u32),
+                ("reserved", u32 * 7)]
+
+
+SERVER_DISCARD_LABELS = 1
 
 
 class QueryResult(C.Structure):

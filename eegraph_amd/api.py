@@ -251,6 +251,64 @@ def run_packed(graphs, algo, sources, opts=None, **kw):
     return outs, st.to_dict()
 
 
+class Server:
+    """Continuous-batching request loop over one graph (eeg_server_*)."""
+
+    def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
+                 ppr_epsilon=0.0, discard_labels=False):
+        cfg = L.ServerCfg()
+        cfg.label_budget_bytes = label_budget_bytes
+        cfg.max_batch = max_batch
+        cfg.K = K
+        cfg.damping = damping
+        cfg.ppr_epsilon = ppr_epsilon
+        cfg.flags = L.SERVER_DISCARD_LABELS if discard_labels else 0
+        self.graph = graph  # keeps the graph alive while the server runs
+        self.n = graph.info()["n"]
+        self.discard = discard_labels
+        h = C.c_void_p()
+        L.check(L.lib.eeg_server_create(graph.handle, C.byref(cfg), C.byref(h)), "eeg_server_create")
+        self._h = h
+
+    def submit(self, algo, source=0):
+        a = L.ALGOS[algo] if isinstance(algo, str) else int(algo)
+        t = C.c_uint64(0)
+        L.check(L.lib.eeg_server_submit(self._h, a, int(source), C.byref(t)), "eeg_server_submit")
+        return t.value
+
+    def wait(self, ticket, want_labels=True, timeout_ms=-1.0, algo=None):
+        """Returns (labels or None, result dict)."""
+        out = None
+        if want_labels and not self.discard:
+            out = np.empty(self.n, np.float32 if algo in ("pr", "ppr", L.PR, L.PPR) else np.uint32)
+        r = L.QueryResult()
+        rc = L.lib.eeg_server_wait(self._h, ticket, _ptr(out), timeout_ms, C.byref(r))
+        L.check(rc, "eeg_server_wait")
+        return out, {"algo": r.algo, "source": r.source, "batch_size": r.batch_size,
+                     "status": r.status, "t_submit_ms": r.t_submit_ms,
+                     "t_start_ms": r.t_start_ms, "t_done_ms": r.t_done_ms}
+
+    def cancel(self, ticket):
+        L.check(L.lib.eeg_server_cancel(self._h, ticket), "eeg_server_cancel")
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            L.check(L.lib.eeg_server_destroy(self._h), "eeg_server_destroy")
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def energy_read(device=0):
     """Device energy accumulator in microjoules, or None if amd-smi is unavailable."""
     uj = C.c_double(0)

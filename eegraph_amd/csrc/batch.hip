@@ -17,12 +17,12 @@
 #include <cstring>
 #include <vector>
 
-#include "engine.hpp"
+#include "batch.hpp"
 
 namespace eeg {
 namespace {
 
-constexpr uint32_t MAXW = 4;  // up to 256 queries per pass (4 x 64-bit words)
+constexpr uint32_t MAXW = BATCH_MAXW;
 
 unsigned grid_n(uint64_t work, unsigned cap = 16384) {
     uint64_t g = (work + WG - 1) / WG;
@@ -32,28 +32,6 @@ unsigned grid_n(uint64_t work, unsigned cap = 16384) {
 
 
 // ---------------------------------------------------------------- multi-source BFS
-struct MsArgs {
-    const uint64_t* row;
-    const uint32_t* col;
-    const uint32_t* vnc;
-    const uint32_t* qin;   // frontier vertices
-    const uint32_t* sin;   // their virtual-node prefix
-    uint64_t* seen;        // n x W
-    uint64_t* visit;       // n x W: bits a frontier vertex pushes this level
-    uint64_t* next;        // n x W: bits received this level
-    uint32_t* stamp;
-    uint32_t* cand;        // vertices that received bits (finalize list)
-    uint32_t* cand_s;
-    uint32_t* qout;        // next frontier
-    uint32_t* sout;
-    uint32_t* dist;        // n x B levels, vertex-major
-    Ctl* ctl;
-    uint32_t W, B, n;
-    uint32_t launch;       // launch index (slot rotation)
-    uint32_t level;        // level being discovered by this push (1-based)
-    uint32_t stampv;
-};
-
 template <int K>
 __global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
     constexpr int VPR = WG / K, TV = TILE_EDGES / K, ROUNDS = TILE_EDGES / WG, EMAX = TV + 2;
@@ -65,6 +43,7 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
     __shared__ uint32_t sQ[TILE_EDGES];
     __shared__ uint32_t sMisc[8];
     Ctl* ctl = a.ctl;
+    if (a.act && blockIdx.x == 0 && threadIdx.x < MAXW) a.act[(a.level % 4) * MAXW + threadIdx.x] = 0;
     if (ctl->done) return;
     const unsigned long long pk = ctl->qpack[(a.launch - 1) % 3];
     const uint32_t nE = (uint32_t)(pk & 0xffffffffull), T = (uint32_t)(pk >> 32);
@@ -160,7 +139,7 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
             }
             __syncthreads();
             const uint32_t bq = sMisc[3];
-            if ((uint64_t)bq + cnt <= a.n) {
+            if ((uint64_t)bq + cnt <= a.qcap) {
                 for (uint32_t i = threadIdx.x; i < cnt; i += WG) a.cand[bq + i] = sQ[i];
             } else if (threadIdx.x == 0) {
                 atomicOr(&ctl->err, 4u);
@@ -176,8 +155,11 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
 __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
     __shared__ uint64_t sScan[WG / WAVE + 1];
     __shared__ unsigned long long sBase;
+    __shared__ unsigned long long sAct[MAXW];
     Ctl* ctl = a.ctl;
     if (ctl->done) return;
+    if (threadIdx.x < MAXW) sAct[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t count = (uint32_t)(ctl->qpack[(a.launch - 1) % 3] & 0xffffffffull);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (count == 0) {
@@ -192,6 +174,7 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
         if (i < count) {
             u = a.cand[i];
             bool any = false;
+            const bool pushes = a.vnc[u] != 0;
             for (uint32_t w = 0; w < a.W; ++w) {
                 const uint64_t idx = (uint64_t)u * a.W + w;
                 const uint64_t nw = a.next[idx] & ~a.seen[idx];
@@ -200,6 +183,7 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
                 if (nw) {
                     a.seen[idx] |= nw;
                     any = true;
+                    if (a.act && pushes) atomicOr(&sAct[w], (unsigned long long)nw);
                     uint64_t bits = nw;
                     while (bits) {
                         const int b = __ffsll((unsigned long long)bits) - 1;
@@ -220,7 +204,7 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
         __syncthreads();
         if (mine) {
             const uint32_t pos = (uint32_t)(sBase & 0xffffffffull) + (uint32_t)(pre & 0xffffffffull);
-            if (pos < a.n) {
+            if (pos < a.qcap) {
                 a.qout[pos] = u;
                 a.sout[pos] = (uint32_t)(sBase >> 32) + (uint32_t)(pre >> 32);
             } else {
@@ -229,29 +213,11 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
         }
         __syncthreads();
     }
+    if (a.act && threadIdx.x < a.W && sAct[threadIdx.x])
+        atomicOr((unsigned long long*)&a.act[(a.level % 4) * MAXW + threadIdx.x], sAct[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------- batched SSSP / SSWP
-struct PlaneArgs {
-    const uint64_t* row;
-    const uint32_t* col;
-    const uint32_t* wgt;
-    const uint32_t* vnc;
-    const uint32_t* qin;
-    const uint32_t* sin;
-    uint32_t* lab;        // n x B, vertex-major
-    uint64_t* chg;        // n x W: queries whose label improved and were not pushed yet
-    uint64_t* nchg;       // n x W: improvements of this iteration
-    uint32_t* stamp;
-    uint32_t* qout;
-    uint32_t* sout;
-    Ctl* ctl;
-    uint32_t W, B, n;
-    uint32_t it;
-    uint32_t stampv;
-    int widest;           // 0: SSSP (min, +w), 1: SSWP (max, min(.,w))
-};
-
 // One wavefront per edge and 64-query chunk.  Each wavefront walks the edges of the tile's
 // virtual nodes; lane q of the wavefront handles query (chunk*64 + q).
 template <int K>
@@ -265,6 +231,7 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
     __shared__ uint32_t sMisc[8];
     __shared__ uint64_t sScan[WG / WAVE + 1];
     Ctl* ctl = a.ctl;
+    if (a.act && blockIdx.x == 0 && threadIdx.x < MAXW) a.act[(a.it % 4) * MAXW + threadIdx.x] = 0;
     if (ctl->done) return;
     const unsigned long long pk = ctl->qpack[(a.it - 1) % 3];
     const uint32_t nE = (uint32_t)(pk & 0xffffffffull), T = (uint32_t)(pk >> 32);
@@ -362,8 +329,9 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
                     const uint64_t um = __ballot(eup[e]);
                     if (um && lane == 0) {
                         const uint32_t u = eu[e];
-                        atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
-                        // queue vertices that have out-edges (each queued entry owns >= 1 vnode)
+                        // only vertices with out-edges carry changes and are queued (each
+                        // queued entry owns >= 1 virtual node); a sink's label is final
+                        if (a.vnc[u]) atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
                         if (a.vnc[u] && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
                             const uint32_t pos = atomicAdd(&sMisc[2], 1u);
                             if (pos < TILE_EDGES) sQ[pos] = u;
@@ -394,7 +362,7 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
             for (uint32_t i = 0; i < per; ++i) {
                 if (b0 + i >= cnt) break;
                 const uint32_t u = sQ[b0 + i], nv = a.vnc[u];
-                if (bq < a.n) {
+                if (bq < a.qcap) {
                     a.qout[bq] = u;
                     a.sout[bq] = bv;
                 } else {
@@ -413,19 +381,27 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
 // iteration take theirs from nchg.  Runs over the previous frontier, then the new one.
 __global__ void k_plane_rotate(const uint32_t* __restrict__ q, const Ctl* ctl, int slot,
                                uint64_t* __restrict__ from, uint64_t* __restrict__ to, uint32_t W,
-                               int move) {
+                               int move, uint64_t* __restrict__ act) {
+    __shared__ unsigned long long sAct[MAXW];
     if (ctl->done) return;
+    if (threadIdx.x < MAXW) sAct[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t count = (uint32_t)(ctl->qpack[slot] & 0xffffffffull);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)count * W;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t idx = (uint64_t)q[i / W] * W + (i % W);
         if (move) {
-            to[idx] = from[idx];
+            const uint64_t m = from[idx];
+            to[idx] = m;
             from[idx] = 0;
+            if (act && m) atomicOr(&sAct[i % W], (unsigned long long)m);
         } else {
             to[idx] = 0;
         }
     }
+    __syncthreads();
+    if (act && threadIdx.x < W && sAct[threadIdx.x])
+        atomicOr((unsigned long long*)&act[threadIdx.x], sAct[threadIdx.x]);
 }
 
 // out[q * n + u] = planes[perm[u] * B + q]: answers in query-major, original vertex order
@@ -473,6 +449,85 @@ __global__ void k_set_frontier(const uint32_t* __restrict__ fv, const uint32_t* 
         ctl->done = 0;
         ctl->iters = 0;
         ctl->err = 0;
+    }
+}
+
+// Admission of new queries between iterations (one workgroup); see launch_admit.
+__global__ __launch_bounds__(WG) void k_admit(Ctl* ctl, int in_slot, const uint32_t* __restrict__ pv,
+                                              const uint32_t* __restrict__ pc, uint32_t npairs,
+                                              const uint32_t* __restrict__ uniq, uint32_t nuniq,
+                                              const uint32_t* __restrict__ vnc, uint32_t W, uint32_t B,
+                                              uint64_t* m_always, uint64_t* m_edges, uint32_t* lab, uint32_t val,
+                                              uint32_t* __restrict__ q, uint32_t* __restrict__ s, uint32_t qcap) {
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ unsigned long long sBase;
+    if (threadIdx.x == 0 && ctl->done) {  // drained engine: start from empty slots
+        for (int k = 0; k < 3; ++k) ctl->qpack[k] = 0;
+        ctl->done = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < npairs; i += WG) {
+        const uint32_t v = pv[i], c = pc[i];
+        lab[(uint64_t)v * B + c] = val;
+        const uint64_t idx = (uint64_t)v * W + c / 64;
+        const unsigned long long bit = 1ull << (c % 64);
+        if (m_always) atomicOr((unsigned long long*)&m_always[idx], bit);
+        if (m_edges && vnc[v]) atomicOr((unsigned long long*)&m_edges[idx], bit);
+    }
+    for (uint32_t b = 0; b < nuniq; b += WG) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t u = 0, nv = 0;
+        if (i < nuniq) {
+            u = uniq[i];
+            nv = vnc[u];
+        }
+        const uint64_t mine = nv ? (1ull | ((uint64_t)nv << 32)) : 0ull;
+        uint64_t tot;
+        const uint64_t pre = block_excl_scan<uint64_t>(mine, sScan, &tot);
+        if (threadIdx.x == 0) sBase = tot ? atomicAdd(&ctl->qpack[in_slot], (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        if (mine) {
+            const uint32_t pos = (uint32_t)(sBase & 0xffffffffull) + (uint32_t)(pre & 0xffffffffull);
+            if (pos < qcap) {
+                q[pos] = u;
+                s[pos] = (uint32_t)(sBase >> 32) + (uint32_t)(pre >> 32);
+            } else {
+                atomicOr(&ctl->err, 64u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Frees columns: their label entries go back to `fill` and their bits are cleared.
+__global__ void k_release(uint32_t n, uint32_t W, uint32_t B, uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                          uint32_t* __restrict__ lab, uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2) {
+    const uint64_t clr[MAXW] = {c0, c1, c2, c3};
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        for (uint32_t w = 0; w < W; ++w) {
+            uint64_t bits = clr[w];
+            if (!bits) continue;
+            const uint64_t idx = v * W + w;
+            if (m0) m0[idx] &= ~bits;
+            if (m1) m1[idx] &= ~bits;
+            if (m2) m2[idx] &= ~bits;
+            while (bits) {
+                const int b = __ffsll((unsigned long long)bits) - 1;
+                bits &= bits - 1;
+                lab[v * B + w * 64 + (uint32_t)b] = fill;
+            }
+        }
+    }
+}
+
+__global__ void k_extract(const uint32_t* __restrict__ lab, uint32_t B, uint32_t c,
+                          const uint32_t* __restrict__ perm, uint32_t n, uint32_t offset,
+                          uint32_t* __restrict__ out) {
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = lab[(uint64_t)perm[u] * B + c];
+        out[u] = (x < INF) ? x - offset : x;
     }
 }
 
@@ -598,12 +653,14 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     Ctl hc{};
     uint32_t launches = 0, level = 0;
     const uint32_t max_iters = o.max_iters ? o.max_iters : n + 2;
-    // BFS: launches alternate push (odd j) and finalize (even j); planes: one push per
-    // iteration plus two rotation passes.
+    // BFS: one level = push + finalize; planes: one iteration = push + two mask rotations.
+    // Chunks of 8 run ahead of the host's look at the control block.
+    const uint32_t qcap = n + 1;
     uint32_t j = 1;
     while (true) {
         const uint32_t chunk = 8;
-        for (uint32_t c = 0; c < chunk; ++c) {
+        for (uint32_t c = 0; c < chunk; ++c, ++j, ++launches) {
+            const int cur = (int)((j - 1) & 1);
             if (bfs) {
                 MsArgs ma{};
                 ma.row = g->out.rowp();
@@ -619,23 +676,14 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
                 ma.W = W;
                 ma.B = B;
                 ma.n = n;
-                const uint32_t lvl = (j + 1) / 2;
-                ma.level = lvl;
-                ma.launch = j;
-                ma.stampv = base + lvl;
-                const int cur = (int)((lvl - 1) & 1);
+                ma.qcap = qcap;
+                ma.level = j;
+                ma.stampv = base + j;
                 ma.qin = bw.q[cur].as<uint32_t>();
                 ma.sin = bw.s[cur].as<uint32_t>();
                 ma.qout = bw.q[cur ^ 1].as<uint32_t>();
                 ma.sout = bw.s[cur ^ 1].as<uint32_t>();
-                if (j & 1) {
-                    void* args[] = {&ma};
-                    auto fn = (K == 4) ? k_ms_push<4> : (K == 8) ? k_ms_push<8> : (K == 16) ? k_ms_push<16> : k_ms_push<32>;
-                    EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
-                } else {
-                    k_ms_finalize<<<grid, WG, 0, s>>>(ma);
-                    EEG_HIP(hipGetLastError());
-                }
+                EEG_TRY(launch_ms_level(ma, K, grid, s));
             } else {
                 PlaneArgs pa{};
                 pa.row = g->out.rowp();
@@ -650,33 +698,23 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
                 pa.W = W;
                 pa.B = B;
                 pa.n = n;
+                pa.qcap = qcap;
                 pa.it = j;
                 pa.stampv = base + j;
                 pa.widest = widest ? 1 : 0;
-                const int cur = (int)((j - 1) & 1);
                 pa.qin = bw.q[cur].as<uint32_t>();
                 pa.sin = bw.s[cur].as<uint32_t>();
                 pa.qout = bw.q[cur ^ 1].as<uint32_t>();
                 pa.sout = bw.s[cur ^ 1].as<uint32_t>();
-                void* args[] = {&pa};
-                auto fn = (K == 4) ? k_plane_push<4> : (K == 8) ? k_plane_push<8> : (K == 16) ? k_plane_push<16> : k_plane_push<32>;
-                EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
-                // forget the pushed frontier's masks, then move this iteration's improvements in
-                k_plane_rotate<<<2048, WG, 0, s>>>(bw.q[cur].as<uint32_t>(), ctl, (int)((j - 1) % 3),
-                                                    nullptr, bw.m1.as<uint64_t>(), W, 0);
-                k_plane_rotate<<<2048, WG, 0, s>>>(bw.q[cur ^ 1].as<uint32_t>(), ctl, (int)(j % 3),
-                                                    bw.m3.as<uint64_t>(), bw.m1.as<uint64_t>(), W, 1);
-                EEG_HIP(hipGetLastError());
+                EEG_TRY(launch_plane_iteration(pa, K, grid, s));
             }
-            ++j;
-            ++launches;
         }
         EEG_HIP(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
         EEG_HIP(hipStreamSynchronize(s));
         if (hc.err) return fail(EEG_ERR_DEVICE, "batched frontier overflow (err=%u)", hc.err);
         if (hc.done) break;
         level = hc.iters;
-        if (j > 2 * max_iters + 16) return fail(EEG_ERR_NOT_CONVERGED, "batched traversal did not converge");
+        if (j > max_iters + 16) return fail(EEG_ERR_NOT_CONVERGED, "batched traversal did not converge");
     }
     EEG_HIP(hipEventRecord(e1, s));
     bw.epoch = base + j + 2;
@@ -711,5 +749,55 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     }
     return EEG_OK;
 }
+
+eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s) {
+    a.launch = 2 * a.level - 1;
+    void* args[] = {&a};
+    auto fn = (K == 4) ? k_ms_push<4> : (K == 8) ? k_ms_push<8> : (K == 16) ? k_ms_push<16> : k_ms_push<32>;
+    EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
+    a.launch = 2 * a.level;
+    k_ms_finalize<<<grid, WG, 0, s>>>(a);
+    EEG_HIP(hipGetLastError());
+    return EEG_OK;
+}
+
+eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s) {
+    void* args[] = {&a};
+    auto fn = (K == 4) ? k_plane_push<4> : (K == 8) ? k_plane_push<8> : (K == 16) ? k_plane_push<16> : k_plane_push<32>;
+    EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
+    // forget the pushed frontier's masks, then move this iteration's improvements in
+    k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr);
+    k_plane_rotate<<<2048, WG, 0, s>>>(a.qout, a.ctl, (int)(a.it % 3), a.nchg, a.chg, a.W, 1,
+                                        a.act ? a.act + (a.it % 4) * MAXW : nullptr);
+    EEG_HIP(hipGetLastError());
+    return EEG_OK;
+}
+
+eeg_status launch_admit(Ctl* ctl, int in_slot, const uint32_t* pv, const uint32_t* pc, uint32_t npairs,
+                        const uint32_t* uniq, uint32_t nuniq, const uint32_t* vnc, uint32_t W, uint32_t B,
+                        uint64_t* m_always, uint64_t* m_edges, uint32_t* lab, uint32_t val, uint32_t* q,
+                        uint32_t* s, uint32_t qcap, hipStream_t st) {
+    k_admit<<<1, WG, 0, st>>>(ctl, in_slot, pv, pc, npairs, uniq, nuniq, vnc, W, B, m_always, m_edges, lab, val,
+                              q, s, qcap);
+    EEG_HIP(hipGetLastError());
+    return EEG_OK;
+}
+
+eeg_status launch_release(uint32_t n, uint32_t W, uint32_t B, const uint64_t clr[BATCH_MAXW], uint32_t* lab,
+                          uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2, hipStream_t st) {
+    k_release<<<grid_n(n, 65536), WG, 0, st>>>(n, W, B, clr[0], W > 1 ? clr[1] : 0, W > 2 ? clr[2] : 0,
+                                                W > 3 ? clr[3] : 0, lab, fill, m0, m1, m2);
+    EEG_HIP(hipGetLastError());
+    return EEG_OK;
+}
+
+eeg_status launch_extract(const uint32_t* lab, uint32_t B, uint32_t c, const uint32_t* perm, uint32_t n,
+                          uint32_t offset, uint32_t* out, hipStream_t st) {
+    k_extract<<<grid_n(n, 65536), WG, 0, st>>>(lab, B, c, perm, n, offset, out);
+    EEG_HIP(hipGetLastError());
+    return EEG_OK;
+}
+
+int batch_cu_count(int device) { return cu_count_b(device); }
 
 }  // namespace eeg

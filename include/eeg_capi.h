@@ -244,17 +244,28 @@ eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sour
                         uint32_t nsample, eeg_opts* best, double* best_ms);
 
 /* ---------------- serving ----------------
- * A request loop over one device-resident graph: requests are admitted while their label
- * memory fits label_budget_bytes, batched with whatever else is in flight (continuous
- * batching) and answered with the same labels as eeg_run. */
+ * No counterpart in the reference (one query per process, src/main.cu:101-121).  A request
+ * loop over one device-resident graph with continuous batching: BFS, SSSP and SSWP queries
+ * take a column of their engine's label plane when one is free (admission by label memory)
+ * and join the next iteration of the traversal already running; a column is given back when
+ * its answer has been read (or at convergence with EEG_SERVER_DISCARD_LABELS).  PPR, PR and CC
+ * run one at a time on a second lane that overlaps with the engines.  Every answer equals
+ * eeg_run's for the same query.  The graph must outlive the server and must not be destroyed
+ * while it runs; eeg_run calls on the same graph from other threads are allowed. */
 typedef struct eeg_server_cfg {
-    uint64_t label_budget_bytes; /* HBM reserved for in-flight label planes */
-    uint32_t max_batch;          /* queries per traversal (<= 256) */
-    uint32_t K;                  /* 0 = autotuned/default */
+    uint64_t label_budget_bytes; /* HBM for the engines' label planes (0 = min(free/4, 48 GiB));
+                                    split evenly over the BFS, SSSP and SSWP engines, each
+                                    column costing 4n bytes of labels + 3n/8 bytes of masks */
+    uint32_t max_batch;          /* columns (queries in flight) per engine, <= 256 (0 = 256) */
+    uint32_t K;                  /* virtual-node degree bound (0 = 8) */
     float damping;               /* PPR damping (0 = 0.85) */
     double ppr_epsilon;          /* PPR residual mass stop (0 = 1e-7) */
-    uint32_t reserved[8];
+    uint32_t flags;              /* EEG_SERVER_DISCARD_LABELS */
+    uint32_t reserved[7];
 } eeg_server_cfg;
+/* Answers are not kept: columns are given back as soon as a query converges and wait() reports
+ * status and times only (labels_out must be NULL).  For throughput runs. */
+#define EEG_SERVER_DISCARD_LABELS 1u
 typedef struct eeg_query_result {
     uint32_t algo;
     uint32_t source;
@@ -269,9 +280,12 @@ eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg, eeg_server
  * can never fit the budget. */
 eeg_status eeg_server_submit(eeg_server* s, uint32_t algo, uint32_t source, uint64_t* ticket);
 /* Blocks until the ticket is done (timeout_ms < 0 = forever); copies its labels (n u32 or
- * f32) into labels_out if not NULL and fills res. */
+ * f32) into labels_out if not NULL, fills res and forgets the ticket (its column is given
+ * back).  A traversal query holds its column until it is waited for or cancelled. */
 eeg_status eeg_server_wait(eeg_server* s, uint64_t ticket, void* labels_out, double timeout_ms,
                            eeg_query_result* res);
+/* Drops a queued query; a running one is forgotten and its column given back when it
+ * converges; a finished one is forgotten.  A PPR/PR/CC query already running completes. */
 eeg_status eeg_server_cancel(eeg_server* s, uint64_t ticket);
 eeg_status eeg_server_destroy(eeg_server* s);
 
