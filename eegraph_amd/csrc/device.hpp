@@ -85,7 +85,7 @@ __device__ __forceinline__ uint32_t wg_search(const uint32_t* __restrict__ S, ui
 // Relaxation rules.  cand() returns the label a neighbour would get over an edge of weight w
 // from a source labelled `lab`; ok is false when the source cannot improve anything (BFS/SSSP
 // relax only from finite labels, which removes the INF+w wrap of src/sssp.cu:62).
-enum Algo : int { A_BFS = 0, A_SSSP = 1, A_SSWP = 2, A_CC = 3, A_PRPUSH = 4 };
+enum Algo : int { A_BFS = 0, A_SSSP = 1, A_SSWP = 2, A_CC = 3 };
 
 template <int ALG>
 struct Rule;

@@ -90,7 +90,6 @@ struct Work {
     DevMem vnpush;      // u32[P] last label pushed by each virtual node (async dd)
     DevMem ctl;         // Ctl
     DevMem fx[2], fl[2], fc[2];  // PR: ranks as (hi, lo) pairs and contributions x/outdeg
-    DevMem rres, push;  // PR residual push: residuals and per-vertex push amounts
     DevMem spill;       // PR: per-tile partial sums
     DevMem red;         // PR: per-workgroup residual partials
     uint32_t epoch = 1; // iteration stamps already used on stamp/flag
@@ -163,9 +162,4 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
                          eeg_stats* st, const uint32_t* d_seeds = nullptr, uint32_t nseeds = 0);
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st);
-eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t K, const uint32_t* q,
-                               const uint32_t* sq, const float* push, float* r, float thr,
-                               uint32_t* cand_out, uint32_t* cand_s_out, uint32_t cap, Ctl* ctl,
-                               uint32_t launch_it, int count_stats);
-
 }  // namespace eeg

@@ -13,10 +13,12 @@
 //   async variants - the same gather with ranks updated in place (Gauss-Seidel in tile order):
 //                    fewer sweeps, no atomics, order of reads not fixed (result within the
 //                    tolerance, not bit-reproducible).
-//   personalised   - residual (delta) push from a one-hot residual (1-d) at the source:
-//                    x_v += r_v (compensated), d*r_v/deg pushed to the out-neighbours' residual
-//                    with float atomics until every residual is below a threshold, so only the
-//                    source's neighbourhood is touched.
+//   personalised   - the same iterations with a one-hot restart: x0 = e_s and
+//                    x' = (1-d) e_s + d * sum_{u->v} x_u/outdeg(u), stopped at the same L1
+//                    residual.  A residual push with a per-vertex threshold small enough for
+//                    an L1 bound of 1e-7 at N = 2^26 touches every vertex many times with float
+//                    atomics; the gather does each iteration in one pass without atomics.
+//   Mass of vertices without out-edges is dropped (not redistributed) in both forms.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -74,9 +76,11 @@ struct PullArgs {
     float2* spill;           // per tile: head and tail partials
     double* res_part;        // per workgroup FP64 residual
     Ctl* ctl;
-    float base;              // (1-d)/N
+    float base;              // teleport term of every vertex: (1-d)/N, or 0 when personalised
     float damping;
     uint32_t it;
+    uint32_t pers;           // personalised: restart vertex (internal id), else 0xffffffff
+    float pers_base;         // its teleport term, 1-d
 };
 
 // New rank base + d * sum kept as an FP32 pair (hi, lo): the product d*s is split exactly with
@@ -94,7 +98,7 @@ __device__ __forceinline__ void rank_pair(float base, float d, Comp sum, float* 
 
 __device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Comp sum, double& res) {
     float hi, lo;
-    rank_pair(a.base, a.damping, sum, &hi, &lo);
+    rank_pair(v == a.pers ? a.pers_base : a.base, a.damping, sum, &hi, &lo);
     const double xo = (double)a.x_old[v] + (double)a.xl_old[v];
     a.x_new[v] = hi;
     a.xl_new[v] = lo;
@@ -255,17 +259,19 @@ __global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __res
     }
 }
 
-// Vertices without in-edges hold (1-d)/N from the first iteration on.
+// Vertices without in-edges hold their teleport term from the first iteration on.
 __global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint64_t* __restrict__ out_row,
                               uint32_t n, const float* __restrict__ x_old, const float* __restrict__ xl_old,
                               float* __restrict__ x_new, float* __restrict__ xl_new,
-                              float* __restrict__ c_new, float base, double* res_part, const Ctl* ctl) {
+                              float* __restrict__ c_new, float base0, uint32_t pers, float pers_base,
+                              double* res_part, const Ctl* ctl) {
     __shared__ double sRed[WG / WAVE];
     if (ctl->done) return;
     double res = 0;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         if (in_row[v + 1] != in_row[v]) continue;
+        const float base = (v == pers) ? pers_base : base0;
         x_new[v] = base;
         xl_new[v] = 0.0f;
         const uint64_t d = out_row[v + 1] - out_row[v];
@@ -302,14 +308,16 @@ __global__ void k_pr_reduce(const double* __restrict__ part, uint32_t nparts, Ct
     }
 }
 
-__global__ void k_pr_init(const uint64_t* __restrict__ out_row, uint32_t n, float x0, float* __restrict__ x,
-                          float* __restrict__ xl, float* __restrict__ c) {
+// x0 = 1/N everywhere, or e_s when personalised (pers = s)
+__global__ void k_pr_init(const uint64_t* __restrict__ out_row, uint32_t n, float x0, uint32_t pers,
+                          float* __restrict__ x, float* __restrict__ xl, float* __restrict__ c) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
-        x[v] = x0;
+        const float xv = (pers == 0xffffffffu) ? x0 : (v == pers ? 1.0f : 0.0f);
+        x[v] = xv;
         xl[v] = 0.0f;
         const uint64_t d = out_row[v + 1] - out_row[v];
-        c[v] = d ? x0 / (float)d : 0.0f;
+        c[v] = d ? xv / (float)d : 0.0f;
     }
 }
 
@@ -329,146 +337,12 @@ __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz,
     }
 }
 
-
-// ---------------------------------------------------------------- residual push
-// Round = take pass + scatter pass.  The take pass consumes every candidate residual above
-// thr (x += r as a compensated pair, push = d*r/outdeg) and queues the vertices with
-// out-edges; the scatter pass (relax kernel, residual rule) adds push[v] to the residual of
-// each out-neighbour and queues the neighbours whose residual crosses thr.
-struct TakeArgs {
-    const uint64_t* row;  // out-edge CSR
-    uint32_t n;
-    const uint32_t* cand; // candidate vertices (nullptr: all vertices 0..n-1)
-    int slot_in;          // Ctl::qpack slot with the candidate count (-1: n)
-    int slot_out;
-    float* r;
-    float* xh;
-    float* xl;
-    float* push;
-    uint32_t* q_out;
-    uint32_t* s_out;
-    Ctl* ctl;
-    float d;
-    float thr;
-    uint32_t K;
-    uint32_t launch;      // launch index (slot rotation)
-    uint32_t round;
-};
-
-constexpr int TAKE_ITEMS = 4;
-
-__global__ __launch_bounds__(WG) void k_pr_take(TakeArgs a) {
-    __shared__ uint64_t sScan[WG / WAVE + 1];
-    __shared__ unsigned long long sBase;
-    __shared__ double sRed[WG / WAVE];
-    Ctl* ctl = a.ctl;
-    if (ctl->done) return;
-    const uint32_t count = (a.slot_in >= 0) ? (uint32_t)(ctl->qpack[a.slot_in] & 0xffffffffull) : a.n;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (count == 0) {
-        if (lead) ctl->done = 1;
-        return;
-    }
-    if (lead) {
-        ctl->qpack[(a.launch + 1) % 3] = 0;
-        ctl->rounds = a.round;
-    }
-    double lost = 0.0;  // residual mass leaving the system this pass (to x or dropped)
-    const uint64_t stride = (uint64_t)gridDim.x * WG * TAKE_ITEMS;
-    for (uint64_t base = (uint64_t)blockIdx.x * WG * TAKE_ITEMS; base < count; base += stride) {
-        uint32_t vv[TAKE_ITEMS], nv[TAKE_ITEMS];
-        uint64_t mine = 0;  // packed (vnodes << 32) | entries of this thread's items
-#pragma unroll
-        for (int j = 0; j < TAKE_ITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)threadIdx.x * TAKE_ITEMS + j;
-            nv[j] = 0;
-            vv[j] = 0;
-            if (i >= count) continue;
-            const uint32_t v = a.cand ? a.cand[i] : (uint32_t)i;
-            vv[j] = v;
-            const float take = a.r[v];
-            if (!(take > a.thr)) continue;
-            a.r[v] = 0.0f;
-            // (xh, xl) += take, renormalised
-            const float h = a.xh[v];
-            const float s = h + take;
-            const float bb = s - h;
-            const float err = (h - (s - bb)) + (take - bb);
-            const float lo = a.xl[v] + err;
-            const float nh = s + lo;
-            a.xh[v] = nh;
-            a.xl[v] = lo - (nh - s);
-            const uint64_t deg = a.row[v + 1] - a.row[v];
-            if (deg) {
-                a.push[v] = (a.d * take) / (float)deg;
-                nv[j] = (uint32_t)((deg + a.K - 1) / a.K);
-                mine += (1ull | ((uint64_t)nv[j] << 32));
-                lost += (double)take * (1.0 - (double)a.d);
-            } else {
-                lost += (double)take;
-            }
-        }
-        uint64_t tot;
-        uint64_t pre = block_excl_scan<uint64_t>(mine, sScan, &tot);
-        if (threadIdx.x == 0)
-            sBase = tot ? atomicAdd(&ctl->qpack[a.slot_out], (unsigned long long)tot) : 0ull;
-        __syncthreads();
-        const unsigned long long b = sBase;
-        uint32_t qpos = (uint32_t)(b & 0xffffffffull) + (uint32_t)(pre & 0xffffffffull);
-        uint32_t spos = (uint32_t)(b >> 32) + (uint32_t)(pre >> 32);
-#pragma unroll
-        for (int j = 0; j < TAKE_ITEMS; ++j) {
-            if (nv[j]) {
-                if (qpos < a.n) {
-                    a.q_out[qpos] = vv[j];
-                    a.s_out[qpos] = spos;
-                } else {
-                    atomicOr(&ctl->err, 2u);
-                }
-                ++qpos;
-                spos += nv[j];
-            }
-        }
-        __syncthreads();
-    }
-    lost = wave_sum(lost);
-    if (lane_id() == 0) sRed[threadIdx.x / WAVE] = lost;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        for (int i = 0; i < WG / WAVE; ++i) t += sRed[i];
-        if (t != 0.0) atomicAdd(&ctl->mass, -t);
-    }
-}
-
-__global__ void k_push_init(uint32_t n, float* __restrict__ r, float* __restrict__ xh, float* __restrict__ xl,
-                            float r0, int personalised, uint32_t source, float r_source,
-                            uint32_t* __restrict__ cand, Ctl* ctl, double mass) {
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-        r[v] = personalised ? (v == source ? r_source : 0.0f) : r0;
-        xh[v] = 0.0f;
-        xl[v] = 0.0f;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->mass = mass;
-        if (personalised) {
-            cand[0] = source;
-            ctl->qpack[0] = 1ull;
-        }
-    }
-}
-
 }  // namespace
-
-static eeg_status run_pagerank_push(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                                    eeg_stats* st);
 
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const bool personalised = (algo == EEG_PPR);
-    if (personalised) return run_pagerank_push(g, algo, source, o, st);
     // async variants update ranks in place (Gauss-Seidel in tile order): a gather may see a
     // neighbour's rank of this sweep or of the last one, so the sum order is not fixed, but the
     // iteration converges in fewer sweeps and needs no atomics
@@ -524,8 +398,9 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     Ctl* ctl = w.ctl.as<Ctl>();
     EEG_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
     const float x0 = 1.0f / (float)n;
-    const float base = (1.0f - d) / (float)n;
-    k_pr_init<<<grid_iso, WG, 0, s>>>(g->out.rowp(), n, x0, w.fx[0].as<float>(), w.fl[0].as<float>(),
+    const float base = personalised ? 0.0f : (1.0f - d) / (float)n;
+    const uint32_t pers = personalised ? source : 0xffffffffu;
+    k_pr_init<<<grid_iso, WG, 0, s>>>(g->out.rowp(), n, x0, pers, w.fx[0].as<float>(), w.fl[0].as<float>(),
                                       w.fc[0].as<float>());
     EEG_HIP(hipGetLastError());
 
@@ -541,6 +416,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     pa.ctl = ctl;
     pa.base = base;
     pa.damping = d;
+    pa.pers = pers;
+    pa.pers_base = 1.0f - d;
     double* red = w.red.as<double>();
     auto pull_fn = (K == 4) ? k_pr_pull<4> : (K == 8) ? k_pr_pull<8> : (K == 16) ? k_pr_pull<16> : k_pr_pull<32>;
 
@@ -574,7 +451,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         const uint32_t iso_iters = inplace ? 1 : 2;  // vertices without in-edges settle at once
         if (it <= iso_iters) {
             k_pr_isolated<<<grid_iso, WG, 0, s>>>(g->in.rowp(), g->out.rowp(), n, pa.x_old, pa.xl_old,
-                                                  pa.x_new, pa.xl_new, pa.c_new, base,
+                                                  pa.x_new, pa.xl_new, pa.c_new, base, pers, 1.0f - d,
                                                   red + grid_pull + grid_fix, ctl);
             EEG_HIP(hipGetLastError());
         } else if (it == iso_iters + 1) {
@@ -665,167 +542,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         st->kernels_timed = kn;
         st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
     }
-    (void)source;
     return rc;
 }
 
-
-// Residual push (async PageRank variants and personalised PageRank).
-static eeg_status run_pagerank_push(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                                    eeg_stats* st) {
-    const auto t_host0 = std::chrono::steady_clock::now();
-    const bool personalised = (algo == EEG_PPR);
-    const uint32_t K = o.K ? o.K : 8;
-    if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
-    const float d = o.damping > 0 ? o.damping : 0.85f;
-    if (!(d > 0 && d < 1)) return fail(EEG_ERR_INVALID_ARG, "damping must be in (0,1)");
-    const uint32_t n = g->n;
-    // leftover residual mass <= n * thr = eps * (1-d), so the L1 error stays below eps
-    const double eps = o.epsilon > 0 ? o.epsilon : 1e-7;
-    const float thr = o.acc > 0 ? o.acc : (float)(eps * (1.0 - d) / (double)n);
-    const uint32_t max_rounds = o.max_iters ? o.max_iters : 100000;
-    hipStream_t s = g->stream;
-    Work& w = g->work;
-    EEG_TRY(ensure_tigr(g, g->out, g->tout, K));
-    EEG_TRY(w.ctl.alloc(sizeof(Ctl)));
-    if (!w.host_ctl) EEG_HIP(hipHostMalloc((void**)&w.host_ctl, 2 * sizeof(Ctl), hipHostMallocDefault));
-    EEG_TRY(w.rres.alloc((size_t)n * 4 + 4));
-    EEG_TRY(w.push.alloc((size_t)n * 4 + 4));
-    EEG_TRY(w.fx[0].alloc((size_t)n * 4 + 4));
-    EEG_TRY(w.fl[0].alloc((size_t)n * 4 + 4));
-    for (int i = 0; i < 2; ++i) {
-        EEG_TRY(w.q[i].alloc((size_t)n * 4 + 4));
-        EEG_TRY(w.s[i].alloc((size_t)n * 4 + 4));
-    }
-    EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
-    Ctl* ctl = w.ctl.as<Ctl>();
-    EEG_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
-    const unsigned grid_n = std::max(1u, std::min(8192u, (n + WG - 1) / WG));
-    const double mass0 = 1.0 - (double)d;
-    k_push_init<<<grid_n, WG, 0, s>>>(n, w.rres.as<float>(), w.fx[0].as<float>(), w.fl[0].as<float>(),
-                                      (float)(mass0 / n), personalised ? 1 : 0, source, (float)mass0,
-                                      w.q[0].as<uint32_t>(), ctl, mass0);
-    EEG_HIP(hipGetLastError());
-
-    TakeArgs ta{};
-    ta.row = g->out.rowp();
-    ta.n = n;
-    ta.r = w.rres.as<float>();
-    ta.xh = w.fx[0].as<float>();
-    ta.xl = w.fl[0].as<float>();
-    ta.push = w.push.as<float>();
-    ta.q_out = w.q[1].as<uint32_t>();
-    ta.s_out = w.s[1].as<uint32_t>();
-    ta.ctl = ctl;
-    ta.d = d;
-    ta.thr = thr;
-    ta.K = K;
-    const unsigned grid_take = (unsigned)cus * 4;
-
-    hipEvent_t ev_start;
-    EEG_TRY(w.event(0, &ev_start));
-    // launch j (1-based): odd = take pass of round (j+1)/2, even = scatter pass
-    auto launch = [&](uint32_t j) -> eeg_status {
-        hipEvent_t eb;
-        if (o.timing) {
-            EEG_TRY(w.event(2 * (size_t)j - 1, &eb));
-            EEG_HIP(hipEventRecord(eb, s));
-        }
-        if (j & 1) {
-            ta.launch = j;
-            ta.round = (j + 1) / 2;
-            const bool first_all = (j == 1 && !personalised);
-            ta.cand = first_all ? nullptr : w.q[0].as<uint32_t>();
-            ta.slot_in = first_all ? -1 : (int)((j - 1) % 3);
-            ta.slot_out = (int)(j % 3);
-            k_pr_take<<<grid_take, WG, 0, s>>>(ta);
-            EEG_HIP(hipGetLastError());
-        } else {
-            EEG_TRY(launch_push_scatter(s, g->device, g->out, K, w.q[1].as<uint32_t>(), w.s[1].as<uint32_t>(),
-                                        w.push.as<float>(), w.rres.as<float>(), thr, w.q[0].as<uint32_t>(),
-                                        w.s[0].as<uint32_t>(), n, ctl, j, o.count_work ? 1 : 0));
-        }
-        hipEvent_t ea;
-        EEG_TRY(w.event(2 * (size_t)j, &ea));
-        EEG_HIP(hipEventRecord(ea, s));
-        return EEG_OK;
-    };
-    EEG_HIP(hipEventRecord(ev_start, s));
-    uint32_t j = 1;
-    eeg_status rc = EEG_OK;
-    {
-        uint32_t chunk = 4;
-        hipEvent_t poll[2];
-        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
-        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
-        auto submit = [&](int slot) -> eeg_status {
-            for (uint32_t k = 0; k < chunk; ++k, ++j) EEG_TRY(launch(j));
-            EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-            EEG_HIP(hipEventRecord(poll[slot], s));
-            chunk = std::min<uint32_t>(chunk * 2, 16);
-            return EEG_OK;
-        };
-        int cur = 0;
-        rc = submit(0);
-        while (rc == EEG_OK) {
-            if (j <= 2 * max_rounds) rc = submit(cur ^ 1);
-            if (rc != EEG_OK) break;
-            EEG_HIP(hipEventSynchronize(poll[cur]));
-            const Ctl& h = w.host_ctl[cur];
-            if (h.done || h.err) break;
-            if (j > 2 * max_rounds) {
-                EEG_HIP(hipEventSynchronize(poll[cur ^ 1]));
-                if (!w.host_ctl[cur ^ 1].done)
-                    rc = fail(EEG_ERR_NOT_CONVERGED, "residual push did not drain after %u rounds", max_rounds);
-                break;
-            }
-            cur ^= 1;
-        }
-        (void)hipEventDestroy(poll[0]);
-        (void)hipEventDestroy(poll[1]);
-    }
-    EEG_HIP(hipStreamSynchronize(s));
-    EEG_HIP(hipMemcpy(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    const Ctl hc = w.host_ctl[0];
-    if (rc == EEG_OK && hc.err) rc = fail(EEG_ERR_DEVICE, "residual push queue overflow (err=%u)", hc.err);
-    k_pr_output<<<grid_n, WG, 0, s>>>(w.fx[0].as<float>(), w.fl[0].as<float>(), n, w.lab.as<float>());
-    EEG_HIP(hipGetLastError());
-    EEG_HIP(hipStreamSynchronize(s));
-    if (st) {
-        // the last launch that did work: take pass of the last round, or the scatter after it
-        const uint32_t last = hc.iters > 2 * hc.rounds - 1 ? hc.iters : 2 * hc.rounds - 1;
-        hipEvent_t ev_end;
-        EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(last, 1), &ev_end));
-        float ms = 0;
-        EEG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end));
-        st->iterations = hc.rounds;
-        st->launches = j - 1;
-        st->ms_traversal = ms;
-        st->pr_residual = hc.mass;
-        st->edges_relaxed = hc.stat[2];
-        st->label_updates = hc.stat[3];
-        st->vertices_touched = hc.stat[0];
-        st->bytes_alg = 12.0 * hc.stat[0] + 12.0 * hc.stat[2] + 8.0 * hc.stat[3];
-        double ksum = 0;
-        uint32_t kn = 0;
-        if (o.timing) {
-            for (uint32_t i = 1; i <= last; ++i) {
-                hipEvent_t eb, ea;
-                EEG_TRY(w.event(2 * (size_t)i - 1, &eb));
-                EEG_TRY(w.event(2 * (size_t)i, &ea));
-                float km = 0;
-                EEG_HIP(hipEventElapsedTime(&km, eb, ea));
-                ksum += km;
-                ++kn;
-            }
-        }
-        st->ms_kernel_sum = ksum;
-        st->kernels_timed = kn;
-        st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
-    }
-    return rc;
-}
 
 }  // namespace eeg

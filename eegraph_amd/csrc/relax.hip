@@ -51,14 +51,11 @@ struct RelaxArgs {
     uint32_t it;        // 1-based iteration of this traversal
     uint32_t stampv;    // stamp value of this iteration
     uint32_t count_stats;
-    float* rres;        // residual push: residual array; `lab` then holds per-vertex push amounts
-    float thr;          // residual push: a vertex is queued when its residual crosses thr
 };
 
 template <int ALG, int K, int MODE>
 __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
-    constexpr bool PUSH = (ALG == A_PRPUSH);
-    using R = Rule<PUSH ? A_CC : ALG>;
+    using R = Rule<ALG>;
     constexpr int VPR = WG / K;             // virtual nodes per round
     constexpr int TV = TILE_EDGES / K;      // virtual nodes per tile
     constexpr int ROUNDS = TILE_EDGES / WG;
@@ -140,7 +137,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                 const uint32_t lab = a.lab[v];
                 uint32_t deg = (uint32_t)(r1 - r0);
                 if (MODE == M_FLAGS && a.flag[v] != a.stampv) deg = 0;
-                if (PUSH ? (lab == 0u) : !R::usable(lab)) deg = 0;
+                if (!R::usable(lab)) deg = 0;
                 sRow[k] = r0;
                 sDeg[k] = deg;
                 sLab[k] = lab;
@@ -185,25 +182,14 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                         const uint64_t e = sRow[k] + eoff;
                         ev[r] = true;
                         ed[r] = a.col[e];
-                        if constexpr (!PUSH && R::weighted) ew[r] = a.wgt[e];
+                        if constexpr (R::weighted) ew[r] = a.wgt[e];
                         el[r] = lab;
                         ++st_edge;
                     }
                 }
             }
             bool eu[ROUNDS];
-            if constexpr (PUSH) {
-                float eo[ROUNDS];
-#pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) {
-                    eo[r] = a.thr;
-                    if (ev[r]) eo[r] = atomicAdd(&a.rres[ed[r]], __uint_as_float(el[r]));
-                }
-                // residual push: exactly one atomic lifts r[d] across thr
-#pragma unroll
-                for (int r = 0; r < ROUNDS; ++r)
-                    eu[r] = ev[r] && eo[r] <= a.thr && eo[r] + __uint_as_float(el[r]) > a.thr;
-            } else {
+            {
                 uint32_t ecur[ROUNDS];
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
@@ -225,22 +211,17 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
             // virtual-node count and a dedup stamp; both are issued for all rounds first.
             uint32_t env[ROUNDS];
             if constexpr (MODE == M_QUEUE) {
-                if constexpr (PUSH) {
 #pragma unroll
-                    for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? 1u : 0u;  // unique crossing
-                } else {
+                for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? a.vnc[ed[r]] : 0u;
+                uint32_t eso[ROUNDS];
 #pragma unroll
-                    for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? a.vnc[ed[r]] : 0u;
-                    uint32_t eso[ROUNDS];
-#pragma unroll
-                    for (int r = 0; r < ROUNDS; ++r) {
-                        eso[r] = a.stampv;
-                        if (env[r]) eso[r] = atomicExch(&a.stamp[ed[r]], a.stampv);
-                    }
-#pragma unroll
-                    for (int r = 0; r < ROUNDS; ++r)
-                        if (eso[r] == a.stampv) env[r] = 0;
+                for (int r = 0; r < ROUNDS; ++r) {
+                    eso[r] = a.stampv;
+                    if (env[r]) eso[r] = atomicExch(&a.stamp[ed[r]], a.stampv);
                 }
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r)
+                    if (eso[r] == a.stampv) env[r] = 0;
             }
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
@@ -465,43 +446,6 @@ int cu_count(int device) {
 }
 
 }  // namespace
-
-// Residual-push scatter: the push queue (entries + virtual-node prefix in Ctl slot
-// `slot_in`) sends push[v] along every out-edge of v; vertices whose residual crosses thr are
-// appended to `cand_out` (count in slot_out).
-eeg_status launch_push_scatter(hipStream_t s, int device, const Csr& c, uint32_t K, const uint32_t* q,
-                               const uint32_t* sq, const float* push, float* r, float thr,
-                               uint32_t* cand_out, uint32_t* cand_s_out, uint32_t cap, Ctl* ctl,
-                               uint32_t launch_it, int count_stats) {
-    RelaxArgs ra{};
-    ra.row = c.rowp();
-    ra.col = c.colp();
-    ra.vnc = nullptr;
-    ra.lab = (uint32_t*)push;
-    ra.qin = q;
-    ra.sin = sq;
-    ra.qout = cand_out;
-    ra.sout = cand_s_out;
-    ra.ctl = ctl;
-    ra.qslot_in = (int)((launch_it - 1) % 3);
-    ra.qslot_out = (int)(launch_it % 3);
-    ra.cap_out = c.n;
-    ra.it = launch_it;
-    ra.stampv = 0;
-    ra.count_stats = count_stats ? 1u : 0u;
-    ra.rres = r;
-    ra.thr = thr;
-    KernelFn fn;
-    switch (K) {
-        case 4: fn = k_relax<A_PRPUSH, 4, M_QUEUE>; break;
-        case 8: fn = k_relax<A_PRPUSH, 8, M_QUEUE>; break;
-        case 16: fn = k_relax<A_PRPUSH, 16, M_QUEUE>; break;
-        default: fn = k_relax<A_PRPUSH, 32, M_QUEUE>; break;
-    }
-    void* args[] = {&ra};
-    EEG_HIP(hipLaunchKernel((const void*)fn, dim3((unsigned)cu_count(device) * 16), dim3(WG), args, 0, s));
-    return EEG_OK;
-}
 
 // Memory the traversal state needs beyond the graph (used by admission control too).
 size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {

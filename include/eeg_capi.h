@@ -89,9 +89,8 @@ typedef struct eeg_opts {
     uint32_t timing;           /* time every relaxation launch with HIP events */
     uint32_t count_work;       /* count entries/edges/updates on the device (adds atomics) */
     float damping;             /* PR/PPR damping (0 = 0.85) */
-    float acc;                 /* PPR: per-vertex residual push threshold (0 = from epsilon) */
-    double epsilon;            /* PR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7);
-                                  PPR: stop when the residual mass < epsilon (0 = 1e-7) */
+    float acc;                 /* unused; kept for the layout (the CLI's --accuracy sets epsilon) */
+    double epsilon;            /* PR and PPR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7) */
     uint32_t wgs_per_cu;       /* resident workgroups per compute unit of the tile kernels
                                   (0 = default: 24 for sweeps, 16 for frontier queues) */
     uint32_t reserved[7];
@@ -259,7 +258,7 @@ typedef struct eeg_server_cfg {
     uint32_t max_batch;          /* columns (queries in flight) per engine, <= 256 (0 = 256) */
     uint32_t K;                  /* virtual-node degree bound (0 = 8) */
     float damping;               /* PPR damping (0 = 0.85) */
-    double ppr_epsilon;          /* PPR residual mass stop (0 = 1e-7) */
+    double ppr_epsilon;          /* PPR stop: L1(x_k+1 - x_k) < ppr_epsilon (0 = 1e-7) */
     uint32_t flags;              /* EEG_SERVER_DISCARD_LABELS */
     uint32_t reserved[7];
 } eeg_server_cfg;

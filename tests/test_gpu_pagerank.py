@@ -101,15 +101,21 @@ def test_pagerank_inplace_vs_fp64(rmat14, variant, K):
     assert np.abs(x.astype(np.float64) - xf).max() < 1e-7
 
 
-def test_personalised_pagerank_vs_fp64(rmat14):
-    """Residual push: leftover residual mass <= eps*(1-d) bounds the L1 error by eps."""
+@pytest.mark.parametrize("variant", ["sync_push_dd", "async_push_dd"])
+def test_personalised_pagerank_vs_fp64(rmat14, variant):
+    """One-hot restart at the source, same iteration and stop rule as PageRank: within the L1
+    tolerance of the FP64 oracle run to 1e-13, and (sync) the same iteration count as the FP64
+    oracle stopped at 1e-7."""
     n, s, d, g, row, col = rmat14
     rng = np.random.default_rng(9)
     for src in list(rng.integers(0, n, 3)) + [0]:
-        x, st = g.run("ppr", int(src), K=8, epsilon=1e-7)
-        assert st["pr_residual"] <= 1e-7 * 0.15 * 1.01
+        x, st = g.run("ppr", int(src), variant=variant, K=8, epsilon=1e-7)
+        assert st["pr_residual"] < 1e-7
         xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(src))
         assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL, f"source {src}"
+        if variant == "sync_push_dd":
+            _, it_o, _ = O.pagerank(n, row, col, 0.85, 1e-7, personalised=True, source=int(src))
+            assert abs(int(st["iterations"]) - int(it_o)) <= 1
 
 
 def test_personalised_pagerank_isolated_source():
