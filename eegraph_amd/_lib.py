@@ -86,10 +86,10 @@ class EdgeList(C.Structure):
 class ServerCfg(C.Structure):
     _fields_ = [("label_budget_bytes", u64), ("max_batch", u32), ("K", u32),
                 ("damping", C.c_float), ("ppr_epsilon", C.c_double), ("flags", u32),
-                ("reserved", u32 * 7)]
+                ("lane_wgs_per_cu", u32), ("lanes", u32), ("reserved", u32 * 5)]
 
 
-SERVER_DISCARD_LABELS = 1
+SERVER_DISCARD_LABELS, SERVER_PLANE_ENGINES, SERVER_NO_BFS_ENGINE = 1, 2, 4
 
 
 class QueryResult(C.Structure):

@@ -255,14 +255,19 @@ class Server:
     """Continuous-batching request loop over one graph (eeg_server_*)."""
 
     def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
-                 ppr_epsilon=0.0, discard_labels=False):
+                 ppr_epsilon=0.0, discard_labels=False, lane_wgs_per_cu=0, lanes=0,
+                 plane_engines=False, bfs_engine=True):
         cfg = L.ServerCfg()
         cfg.label_budget_bytes = label_budget_bytes
         cfg.max_batch = max_batch
         cfg.K = K
         cfg.damping = damping
         cfg.ppr_epsilon = ppr_epsilon
-        cfg.flags = L.SERVER_DISCARD_LABELS if discard_labels else 0
+        cfg.flags = ((L.SERVER_DISCARD_LABELS if discard_labels else 0) |
+                     (L.SERVER_PLANE_ENGINES if plane_engines else 0) |
+                     (0 if bfs_engine else L.SERVER_NO_BFS_ENGINE))
+        cfg.lane_wgs_per_cu = lane_wgs_per_cu
+        cfg.lanes = lanes
         self.graph = graph  # keeps the graph alive while the server runs
         self.n = graph.info()["n"]
         self.discard = discard_labels

@@ -218,10 +218,12 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
 }
 
 // ---------------------------------------------------------------- batched SSSP / SSWP
-// One wavefront per edge and 64-query chunk.  Each wavefront walks the edges of the tile's
-// virtual nodes; lane q of the wavefront handles query (chunk*64 + q).
+// Dense variant: one wavefront per virtual node, every pending 64-query word handled with
+// lane = query, so the label reads and atomics of a wavefront are contiguous 256-byte rows.
+// Best when many queries are in flight together; the two-pass kernel below is better when
+// few are.
 template <int K>
-__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
+__global__ __launch_bounds__(WG) void k_plane_push_wave(PlaneArgs a) {
     constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE;
     __shared__ uint32_t sS[EMAX];
     __shared__ uint64_t sRow[EMAX];
@@ -327,17 +329,260 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
 #pragma unroll
                 for (int e = 0; e < K; ++e) {
                     const uint64_t um = __ballot(eup[e]);
-                    if (um && lane == 0) {
+                    if (um && lane == 0 && a.vnc[eu[e]]) {
                         const uint32_t u = eu[e];
                         // only vertices with out-edges carry changes and are queued (each
                         // queued entry owns >= 1 virtual node); a sink's label is final
-                        if (a.vnc[u]) atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
-                        if (a.vnc[u] && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+                        atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
+                        if (atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
                             const uint32_t pos = atomicAdd(&sMisc[2], 1u);
                             if (pos < TILE_EDGES) sQ[pos] = u;
                             else atomicOr(&ctl->err, 16u);
                         }
                     }
+                }
+            }
+        }
+        __syncthreads();
+        // queue the improved vertices with their virtual-node counts
+        const uint32_t cnt = min(sMisc[2], (uint32_t)TILE_EDGES);
+        if (cnt > 0) {
+            const uint32_t per = (cnt + WG - 1) / WG, b0 = threadIdx.x * per;
+            uint64_t loc = 0;
+            for (uint32_t i = 0; i < per; ++i)
+                if (b0 + i < cnt) loc += 1ull | ((uint64_t)a.vnc[sQ[b0 + i]] << 32);
+            uint64_t tot;
+            uint64_t pre = block_excl_scan<uint64_t>(loc, sScan, &tot);
+            if (threadIdx.x == 0) {
+                const unsigned long long old = atomicAdd(&ctl->qpack[a.it % 3], (unsigned long long)tot);
+                sMisc[3] = (uint32_t)(old & 0xffffffffull);
+                sMisc[4] = (uint32_t)(old >> 32);
+            }
+            __syncthreads();
+            uint32_t bq = sMisc[3] + (uint32_t)(pre & 0xffffffffull);
+            uint32_t bv = sMisc[4] + (uint32_t)(pre >> 32);
+            for (uint32_t i = 0; i < per; ++i) {
+                if (b0 + i >= cnt) break;
+                const uint32_t u = sQ[b0 + i], nv = a.vnc[u];
+                if (bq < a.qcap) {
+                    a.qout[bq] = u;
+                    a.sout[bq] = bv;
+                } else {
+                    atomicOr(&ctl->err, 32u);
+                }
+                ++bq;
+                bv += nv;
+            }
+        }
+        __syncthreads();
+        e_start += (uint32_t)(nnext - 1);
+    }
+}
+
+// One tile of the frontier's virtual nodes in two passes.
+//  Sparse pass: K lanes per virtual node, one edge per lane, as in the single-query kernel.
+//    For every 64-query word with at most SPARSE_MAX pending queries the lane relaxes its edge
+//    for each of them (source and destination label reads issued together).  A virtual node
+//    with a denser word is listed for the dense pass.
+//  Dense pass: one wavefront per listed virtual node.  Lanes fetch its <= K edges and
+//    broadcast them; then for every dense word each lane (one query) issues all K destination
+//    label reads before any atomic, so the reads of a wavefront are contiguous 256-byte rows.
+constexpr uint32_t SPARSE_MAX = 8;
+// queries in flight above which the dense variant is launched
+constexpr uint32_t PLANE_DENSE_FROM = 16;
+
+template <int K>
+__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
+    constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE, VPR = WG / K;
+    constexpr int ROUNDS = TILE_EDGES / WG;
+    __shared__ uint32_t sS[EMAX];
+    __shared__ uint64_t sRow[EMAX];
+    __shared__ uint32_t sDeg[EMAX];
+    __shared__ uint32_t sV[EMAX];
+    __shared__ uint16_t sOwn[TV];
+    __shared__ uint16_t sDense[TV];
+    __shared__ uint32_t sQ[TILE_EDGES];
+    __shared__ uint32_t sMisc[8];
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+    Ctl* ctl = a.ctl;
+    if (a.act && blockIdx.x == 0 && threadIdx.x < MAXW) a.act[(a.it % 4) * MAXW + threadIdx.x] = 0;
+    if (ctl->done) return;
+    const unsigned long long pk = ctl->qpack[(a.it - 1) % 3];
+    const uint32_t nE = (uint32_t)(pk & 0xffffffffull), T = (uint32_t)(pk >> 32);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (nE == 0 || T == 0) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    if (lead) {
+        ctl->qpack[(a.it + 1) % 3] = 0;
+        ctl->iters = a.it;
+    }
+    const uint32_t ntiles = (T + TV - 1) / TV;
+    const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
+    const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+    if (c0 >= c1) return;
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int grp = threadIdx.x / K, lj = threadIdx.x % K;
+    uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+    // an improvement of query q at u: change bit, and u queued once per iteration
+    auto improved = [&](uint32_t u, uint32_t w, uint64_t bits) {
+        if (!a.vnc[u]) return;  // a vertex without out-edges has nothing to push
+        atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)bits);
+        if (atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+            const uint32_t pos = atomicAdd(&sMisc[2], 1u);
+            if (pos < TILE_EDGES) sQ[pos] = u;
+            else atomicOr(&ctl->err, 16u);
+        }
+    };
+    for (uint32_t tile = c0; tile < c1; ++tile) {
+        const uint32_t t0 = tile * TV, t1 = min(t0 + (uint32_t)TV, T);
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint64_t i = (uint64_t)e_start + k;
+            sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
+        }
+        if (threadIdx.x == 0) {
+            sMisc[0] = 1;
+            sMisc[1] = 1;
+            sMisc[2] = 0;
+            sMisc[5] = 0;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < EMAX; k += WG) {
+            const uint32_t cur = sS[k], nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+            if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;
+            if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;
+        }
+        __syncthreads();
+        const int nent = (int)sMisc[0], nnext = (int)sMisc[1];
+        for (int k = threadIdx.x; k < nent; k += WG) {
+            const uint32_t v = a.qin[e_start + k];
+            sV[k] = v;
+            sRow[k] = a.row[v];
+            sDeg[k] = (uint32_t)(a.row[v + 1] - a.row[v]);
+        }
+        for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
+            const uint32_t t = t0 + j;
+            int lo = 0, hi = nent;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (sS[mid] <= t) lo = mid; else hi = mid;
+            }
+            sOwn[j] = (uint16_t)lo;
+        }
+        __syncthreads();
+
+        // ---- sparse pass: edge loads of every round first, then word by word
+        uint32_t rv[ROUNDS], ru[ROUNDS], rw[ROUNDS];
+        bool rin[ROUNDS], rhas[ROUNDS], rdense[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int slot = r * VPR + grp;
+            const uint32_t t = t0 + slot;
+            rin[r] = t < t1;
+            rhas[r] = false;
+            rdense[r] = false;
+            rv[r] = 0;
+            ru[r] = 0;
+            rw[r] = 0;
+            if (rin[r]) {
+                const int k = sOwn[slot];
+                rv[r] = sV[k];
+                const uint32_t eoff = (t - sS[k]) * K + lj;
+                if (eoff < sDeg[k]) {
+                    rhas[r] = true;
+                    ru[r] = a.col[sRow[k] + eoff];
+                    rw[r] = a.wgt[sRow[k] + eoff];
+                }
+            }
+        }
+        for (uint32_t w = 0; w < a.W; ++w) {
+            uint64_t rm[ROUNDS];
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) rm[r] = rin[r] ? a.chg[(uint64_t)rv[r] * a.W + w] : 0ull;
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                uint64_t m = rm[r];
+                if (!m) continue;
+                if ((uint32_t)__popcll((unsigned long long)m) > SPARSE_MAX) {
+                    rdense[r] = true;
+                    continue;
+                }
+                if (!rhas[r]) continue;
+                const uint32_t u = ru[r], wt = rw[r];
+                while (m) {
+                    const uint32_t c = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+                    m &= m - 1;
+                    const uint32_t q = w * 64 + c;
+                    uint32_t* p = &a.lab[(uint64_t)u * a.B + q];
+                    const uint32_t lv = a.lab[(uint64_t)rv[r] * a.B + q];
+                    const uint32_t cur = *p;
+                    if (a.widest ? lv == 0 : lv >= INF) continue;
+                    uint32_t cand;
+                    if (a.widest) cand = lv < wt ? lv : wt;
+                    else {
+                        const uint64_t sm = (uint64_t)lv + wt;
+                        cand = sm < INF ? (uint32_t)sm : INF;
+                    }
+                    if (a.widest ? cand > cur : cand < cur) {
+                        const uint32_t old = a.widest ? atomicMax(p, cand) : atomicMin(p, cand);
+                        if (a.widest ? cand > old : cand < old) improved(u, w, 1ull << c);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+            if (rdense[r] && lj == 0) sDense[atomicAdd(&sMisc[5], 1u)] = (uint16_t)(r * VPR + grp);
+        __syncthreads();
+
+        // ---- dense pass: one wavefront per listed virtual node
+        const uint32_t nd = sMisc[5];
+        for (uint32_t di = (uint32_t)wv; di < nd; di += NW) {
+            const uint32_t slot = sDense[di];
+            const uint32_t t = t0 + slot;
+            const int k = sOwn[slot];
+            const uint32_t v = sV[k];
+            const uint32_t e0 = (t - sS[k]) * K;
+            const uint32_t ne = min((uint32_t)K, sDeg[k] - e0);
+            uint32_t my_u = 0, my_w = 0;
+            if ((uint32_t)lane < ne) {
+                my_u = a.col[sRow[k] + e0 + lane];
+                my_w = a.wgt[sRow[k] + e0 + lane];
+            }
+            for (uint32_t w = 0; w < a.W; ++w) {
+                const uint64_t m = a.chg[(uint64_t)v * a.W + w];
+                if ((uint32_t)__popcll((unsigned long long)m) <= SPARSE_MAX) continue;  // sparse pass did it
+                const uint32_t q = w * 64 + (uint32_t)lane;
+                const bool mine = (m >> lane) & 1ull;
+                const uint32_t lv = mine ? a.lab[(uint64_t)v * a.B + q] : 0u;
+                const bool ok = mine && (a.widest ? lv > 0 : lv < INF);
+                uint32_t eu[K], ecand[K], ecur[K];
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    eu[e] = __shfl(my_u, e, WAVE);
+                    const uint32_t wt = __shfl(my_w, e, WAVE);
+                    if (a.widest) ecand[e] = lv < wt ? lv : wt;
+                    else {
+                        const uint64_t sm = (uint64_t)lv + wt;
+                        ecand[e] = sm < INF ? (uint32_t)sm : INF;
+                    }
+                    ecur[e] = (ok && (uint32_t)e < ne) ? a.lab[(uint64_t)eu[e] * a.B + q] : 0u;
+                }
+                bool eup[K];
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    eup[e] = false;
+                    if (ok && (uint32_t)e < ne && (a.widest ? ecand[e] > ecur[e] : ecand[e] < ecur[e])) {
+                        uint32_t* p = &a.lab[(uint64_t)eu[e] * a.B + q];
+                        const uint32_t old = a.widest ? atomicMax(p, ecand[e]) : atomicMin(p, ecand[e]);
+                        eup[e] = a.widest ? ecand[e] > old : ecand[e] < old;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < K; ++e) {
+                    const uint64_t um = __ballot(eup[e]);
+                    if (um && lane == 0) improved(eu[e], w, um);
                 }
             }
         }
@@ -706,7 +951,7 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
                 pa.sin = bw.s[cur].as<uint32_t>();
                 pa.qout = bw.q[cur ^ 1].as<uint32_t>();
                 pa.sout = bw.s[cur ^ 1].as<uint32_t>();
-                EEG_TRY(launch_plane_iteration(pa, K, grid, s));
+                EEG_TRY(launch_plane_iteration(pa, K, grid, s, nsrc));
             }
         }
         EEG_HIP(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -761,9 +1006,11 @@ eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s) {
     return EEG_OK;
 }
 
-eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s) {
+eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight) {
     void* args[] = {&a};
     auto fn = (K == 4) ? k_plane_push<4> : (K == 8) ? k_plane_push<8> : (K == 16) ? k_plane_push<16> : k_plane_push<32>;
+    if (in_flight > PLANE_DENSE_FROM)
+        fn = (K == 4) ? k_plane_push_wave<4> : (K == 8) ? k_plane_push_wave<8> : (K == 16) ? k_plane_push_wave<16> : k_plane_push_wave<32>;
     EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
     // forget the pushed frontier's masks, then move this iteration's improvements in
     k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr);

@@ -60,9 +60,10 @@ struct PlaneArgs {
 
 // One BFS level: push then finalize (launch indices 2*level-1 and 2*level).
 eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s);
-// One plane iteration: push, then the two change-mask rotations.  q_old is the frontier the
-// push reads, q_new the one it writes.
-eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s);
+// One plane iteration: push, then the two change-mask rotations.  `in_flight` (queries being
+// traversed) picks the push variant: one wavefront per virtual node with lane = query for
+// many, a two-pass kernel with K lanes per virtual node for few.
+eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight);
 
 // Admission between iterations (one workgroup): pairs (pv[i], pc[i]) = (internal source,
 // column) get label `val`; their column bit is ORed into m_always, and into m_edges when the

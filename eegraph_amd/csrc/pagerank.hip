@@ -340,7 +340,7 @@ __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz,
 }  // namespace
 
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                        eeg_stats* st) {
+                        eeg_stats* st, Work* wk, hipStream_t sk) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const bool personalised = (algo == EEG_PPR);
     // async variants update ranks in place (Gauss-Seidel in tile order): a gather may see a
@@ -354,8 +354,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     const double eps = o.epsilon > 0 ? o.epsilon : 1e-7;
     const uint32_t max_iters = o.max_iters ? o.max_iters : 10000;
     const uint32_t n = g->n;
-    hipStream_t s = g->stream;
-    Work& w = g->work;
+    hipStream_t s = wk ? sk : g->stream;
+    Work& w = wk ? *wk : g->work;
     EEG_TRY(ensure_in(g));
     EEG_TRY(ensure_tigr(g, g->in, g->tin, K));
     const Tigr& t = g->tin[k_index(K)];

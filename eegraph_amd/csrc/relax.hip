@@ -457,7 +457,7 @@ size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
 }
 
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                         eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds) {
+                         eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds, Work* wk, hipStream_t sk) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
@@ -480,8 +480,8 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     EEG_TRY(ensure_tigr(g, c, tt, K));
     const Tigr& t = tt[k_index(K)];
     const uint32_t n = g->n;
-    hipStream_t s = g->stream;
-    Work& w = g->work;
+    hipStream_t s = wk ? sk : g->stream;
+    Work& w = wk ? *wk : g->work;
 
     EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
     EEG_TRY(w.ctl.alloc(sizeof(Ctl)));

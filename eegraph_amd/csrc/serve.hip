@@ -19,8 +19,12 @@
 // when every pooled buffer holds an unread answer, further answers go to host memory, so a
 // client that reads answers in any order never blocks admission.
 //
-// PPR, PR and CC go through a single-query lane (one host thread running eeg_run on the
-// graph's own stream), which overlaps with the engines on the device.
+// Queries without an engine go to single-query lanes: each lane is a host thread with its own
+// traversal state and stream, so several such queries run at once and overlap on the device
+// with each other and with the engines.  PPR, PR and CC always take a lane; SSSP and SSWP
+// take the label-plane engines only with EEG_SERVER_PLANE_ENGINES, since planes pay off when
+// many queries sweep the same vertices together and queries admitted at different times
+// rarely do; BFS takes its engine unless EEG_SERVER_NO_BFS_ENGINE.
 //
 // Every answer equals eeg_run's for the same query: the BFS/SSSP/SSWP fixpoints are unique
 // and do not depend on which other columns share the iterations.
@@ -121,6 +125,18 @@ struct Engine {
     ~Engine();
 };
 
+// A single-query lane: a host thread with its own traversal state and stream, so that several
+// queries run at once on one graph and their launches overlap on the device.
+struct Lane {
+    Work w;
+    hipStream_t st = nullptr;
+    DevMem out;
+    std::thread th;
+    ~Lane() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
 }  // namespace
 }  // namespace eeg
 
@@ -135,11 +151,13 @@ struct eeg_server {
     uint64_t next_id = 1;
     eeg::Clock::time_point t0;
     std::unique_ptr<eeg::Engine> eng[3];
+    bool use_eng[3] = {false, false, false};  // algorithms served by a batching engine
     bool stop = false;
-    // single-query lane
+    // single-query lanes
     std::deque<eeg::TicketP> lane_q;
     std::condition_variable lane_cv;
-    std::thread lane_th;
+    std::vector<std::unique_ptr<eeg::Lane>> lanes;
+    std::mutex prep_mu;  // builds of the derived tables a lane needs
     // answer extraction (labels of a finished column to the caller)
     std::mutex io_mu;
     hipStream_t io_st = nullptr;
@@ -263,7 +281,7 @@ eeg_status Engine::launch_iteration(eeg_graph* g, uint32_t j) {
     a.sin = s[cur].as<uint32_t>();
     a.qout = q[cur ^ 1].as<uint32_t>();
     a.sout = s[cur ^ 1].as<uint32_t>();
-    return launch_plane_iteration(a, K, grid, st);
+    return launch_plane_iteration(a, K, grid, st, running);
 }
 
 void fail_running(eeg_server* srv, Engine* e, eeg_status why) {
@@ -298,21 +316,24 @@ void Engine::loop() {
         uint32_t active = 0;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            auto has_release = [&] {
-                return std::find(released.begin(), released.end(), (uint8_t)1) != released.end();
-            };
-            while (!srv->stop && inflight.empty() && running == 0 && !has_release() &&
-                   (pending.empty() || std::find(col.begin(), col.end(), nullptr) == col.end()))
+            auto count_free = [&] { return (size_t)std::count(col.begin(), col.end(), nullptr); };
+            auto count_released = [&] { return (size_t)std::count(released.begin(), released.end(), (uint8_t)1); };
+            while (!srv->stop && inflight.empty() && running == 0 && count_released() == 0 &&
+                   (pending.empty() || count_free() == 0))
                 cv.wait(lk);
             if (srv->stop) break;
-            // columns given back: reset before any reuse (stream order), free for admission now
-            for (uint32_t c = 0; c < ncols; ++c)
-                if (released[c]) {
-                    released[c] = 0;
-                    col[c] = nullptr;
-                    clr[c / 64] |= 1ull << (c % 64);
-                    any_clr = true;
-                }
+            // Columns given back are reset (one pass over the plane for all of them, in stream
+            // order before any reuse) only when waiting queries need them or the engine is idle.
+            const size_t nfree = count_free(), nrel = count_released();
+            if (nrel && (pending.size() > nfree || (running == 0 && pending.empty()))) {
+                for (uint32_t c = 0; c < ncols; ++c)
+                    if (released[c]) {
+                        released[c] = 0;
+                        col[c] = nullptr;
+                        clr[c / 64] |= 1ull << (c % 64);
+                        any_clr = true;
+                    }
+            }
             uint32_t c = 0;
             const double tnow = srv->now_ms();
             while (!pending.empty() && !broken) {
@@ -342,8 +363,11 @@ void Engine::loop() {
         }
         eeg_status rc = EEG_OK;
         if (any_clr) {
-            rc = launch_release(n, W, B, clr, planes.as<uint32_t>(), fill(), m1.as<uint64_t>(),
-                                m2.p ? m2.as<uint64_t>() : nullptr, m3.as<uint64_t>(), st);
+            // a converged plane column has no change bits left; a BFS column keeps its seen
+            // bits and visit bits on vertices without out-edges
+            rc = launch_release(n, W, B, clr, planes.as<uint32_t>(), fill(),
+                                kind == E_BFS ? m1.as<uint64_t>() : nullptr,
+                                kind == E_BFS ? m2.as<uint64_t>() : nullptr, nullptr, st);
         }
         if (rc == EEG_OK && !pv.empty()) {
             uint32_t* hs = hstage + (size_t)(it % 4) * stage_words;
@@ -484,8 +508,24 @@ bool Engine::gather(std::vector<std::pair<TicketP, uint32_t>>& fin) {
     return true;
 }
 
-void lane_loop(eeg_server* srv) {
+// Tables a query reads, built once (under the graph's lock) before any lane uses them.
+eeg_status prepare_for(eeg_server* srv, uint32_t algo) {
     eeg_graph* g = srv->g;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (algo == EEG_CC) {
+        EEG_TRY(ensure_sym(g));
+        return ensure_tigr(g, g->sym, g->tsym, srv->K);
+    }
+    if (algo == EEG_PR || algo == EEG_PPR) {
+        EEG_TRY(ensure_in(g));
+        return ensure_tigr(g, g->in, g->tin, srv->K);
+    }
+    return ensure_tigr(g, g->out, g->tout, srv->K);
+}
+
+void lane_loop(eeg_server* srv, Lane* ln) {
+    eeg_graph* g = srv->g;
+    (void)hipSetDevice(g->device);
     while (true) {
         TicketP tk;
         {
@@ -508,11 +548,27 @@ void lane_loop(eeg_server* srv) {
         memset(&o, 0, sizeof(o));
         o.variant = EEG_SYNC_PUSH_DD;
         o.K = srv->K;
+        // a small resident grid leaves room on every CU for the other lanes and the engines
+        o.wgs_per_cu = srv->cfg.lane_wgs_per_cu ? srv->cfg.lane_wgs_per_cu : 4;
         o.damping = srv->cfg.damping;
         o.epsilon = srv->cfg.ppr_epsilon;
+        const uint32_t a = tk->algo;
+        const bool sourced = a == EEG_BFS || a == EEG_SSSP || a == EEG_SSWP || a == EEG_PPR;
+        const uint32_t isrc = sourced ? g->h_perm[tk->source] : 0u;
         std::vector<uint32_t> out;
-        if (!srv->discard) out.resize(g->n);
-        const eeg_status rc = eeg_run(g, tk->algo, tk->source, &o, srv->discard ? nullptr : out.data(), nullptr);
+        eeg_status rc = prepare_for(srv, a);
+        if (rc == EEG_OK)
+            rc = (a == EEG_PR || a == EEG_PPR) ? run_pagerank(g, a, isrc, o, nullptr, &ln->w, ln->st)
+                                               : run_traversal(g, a, isrc, o, nullptr, nullptr, 0, &ln->w, ln->st);
+        if (rc == EEG_OK && !srv->discard) {  // answer in original vertex order
+            out.resize(g->n);
+            rc = ln->out.alloc((size_t)g->n * 4 + 4);
+            if (rc == EEG_OK) rc = unpermute(g, ln->w.lab.p, ln->out.p, ln->st);
+            if (rc == EEG_OK &&
+                (hipMemcpyAsync(out.data(), ln->out.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, ln->st) != hipSuccess ||
+                 hipStreamSynchronize(ln->st) != hipSuccess))
+                rc = fail(EEG_ERR_DEVICE, "copying the answer failed");
+        }
         std::lock_guard<std::mutex> lk(srv->mu);
         tk->status = rc;
         tk->state = T_DONE;
@@ -540,7 +596,9 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32");
     if (c.damping == 0.0f) c.damping = 0.85f;
     if (c.ppr_epsilon == 0.0) c.ppr_epsilon = 1e-7;
-    if (c.flags & ~(uint32_t)EEG_SERVER_DISCARD_LABELS) return fail(EEG_ERR_INVALID_ARG, "unknown flags 0x%x", c.flags);
+    const uint32_t known = EEG_SERVER_DISCARD_LABELS | EEG_SERVER_PLANE_ENGINES | EEG_SERVER_NO_BFS_ENGINE;
+    if (c.flags & ~known) return fail(EEG_ERR_INVALID_ARG, "unknown flags 0x%x", c.flags);
+    if (c.lanes > 64) return fail(EEG_ERR_INVALID_ARG, "lanes must be <= 64");
     DeviceGuard dg;
     EEG_TRY(select_device(g->device));
     {
@@ -561,15 +619,22 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         c.label_budget_bytes = std::min<uint64_t>(fr / 4, 48ull << 30);
         srv->cfg.label_budget_bytes = c.label_budget_bytes;
     }
+    // engines in use: BFS unless disabled, SSSP/SSWP label planes only when asked for
+    bool use[3] = {(c.flags & EEG_SERVER_NO_BFS_ENGINE) == 0,
+                   g->weighted && (c.flags & EEG_SERVER_PLANE_ENGINES) != 0,
+                   g->weighted && (c.flags & EEG_SERVER_PLANE_ENGINES) != 0};
+    const int neng = (int)use[0] + (int)use[1] + (int)use[2];
     // label budget split evenly over the engines; a column costs its labels plus its share of
     // the bit masks
-    const int neng = g->weighted ? 3 : 1;
     const double per_col = (double)n * 4.0 + (double)n * 8.0 * 3.0 / 64.0;
-    const uint64_t cols = (uint64_t)((double)(c.label_budget_bytes / neng) / std::max(per_col, 1.0));
+    const uint64_t cols =
+        neng ? (uint64_t)((double)(c.label_budget_bytes / neng) / std::max(per_col, 1.0)) : 0;
     uint64_t fit = cols;
     if (fit >= 16 && fit < c.max_batch) fit -= fit % 16;  // 64-byte aligned label rows
     const uint32_t ncols = (uint32_t)std::min<uint64_t>(fit, c.max_batch);
-    for (int k = 0; k < neng && ncols > 0; ++k) {
+    for (int k = 0; k < 3; ++k) srv->use_eng[k] = use[k];
+    for (int k = 0; k < 3 && ncols > 0; ++k) {
+        if (!use[k]) continue;
         srv->eng[k].reset(new (std::nothrow) Engine());
         if (!srv->eng[k]) return fail(EEG_ERR_OOM, "host allocation failed");
         Engine& e = *srv->eng[k];
@@ -581,9 +646,15 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     }
     EEG_HIP(hipStreamCreateWithFlags(&srv->io_st, hipStreamNonBlocking));
     EEG_TRY(srv->io_buf.alloc((size_t)n * 4 + 4));
+    const uint32_t nl = c.lanes ? c.lanes : 4;
+    for (uint32_t i = 0; i < nl; ++i) {
+        srv->lanes.emplace_back(new (std::nothrow) Lane());
+        if (!srv->lanes.back()) return fail(EEG_ERR_OOM, "host allocation failed");
+        EEG_HIP(hipStreamCreateWithFlags(&srv->lanes.back()->st, hipStreamNonBlocking));
+    }
     for (int k = 0; k < 3; ++k)
         if (srv->eng[k]) srv->eng[k]->th = std::thread([e = srv->eng[k].get()] { e->loop(); });
-    srv->lane_th = std::thread([s = srv.get()] { lane_loop(s); });
+    for (auto& ln : srv->lanes) ln->th = std::thread([s = srv.get(), l = ln.get()] { lane_loop(s, l); });
     *out = srv.release();
     return EEG_OK;
 }
@@ -604,7 +675,7 @@ extern "C" eeg_status eeg_server_submit(eeg_server* s, uint32_t algo, uint32_t s
     tk->id = s->next_id++;
     tk->t_submit = s->now_ms();
     const int e = algo == EEG_BFS ? E_BFS : algo == EEG_SSSP ? E_SSSP : algo == EEG_SSWP ? E_SSWP : -1;
-    if (e >= 0) {
+    if (e >= 0 && s->use_eng[e]) {
         Engine* eng = s->eng[e].get();
         if (!eng) return fail(EEG_ERR_REJECTED, "label budget holds no column for this algorithm");
         if (eng->broken) return fail(EEG_ERR_DEVICE, "serving engine stopped after a device error");
@@ -730,7 +801,8 @@ extern "C" eeg_status eeg_server_destroy(eeg_server* s) {
     }
     for (auto& e : s->eng)
         if (e && e->th.joinable()) e->th.join();
-    if (s->lane_th.joinable()) s->lane_th.join();
+    for (auto& ln : s->lanes)
+        if (ln && ln->th.joinable()) ln->th.join();
     for (auto& e : s->eng)
         if (e && e->st) (void)hipStreamSynchronize(e->st);
     if (s->io_st) {

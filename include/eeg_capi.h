@@ -244,13 +244,14 @@ eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sour
 
 /* ---------------- serving ----------------
  * No counterpart in the reference (one query per process, src/main.cu:101-121).  A request
- * loop over one device-resident graph with continuous batching: BFS, SSSP and SSWP queries
- * take a column of their engine's label plane when one is free (admission by label memory)
- * and join the next iteration of the traversal already running; a column is given back when
- * its answer has been read (or at convergence with EEG_SERVER_DISCARD_LABELS).  PPR, PR and CC
- * run one at a time on a second lane that overlaps with the engines.  Every answer equals
- * eeg_run's for the same query.  The graph must outlive the server and must not be destroyed
- * while it runs; eeg_run calls on the same graph from other threads are allowed. */
+ * loop over one device-resident graph.  Engine queries (BFS; SSSP and SSWP with
+ * EEG_SERVER_PLANE_ENGINES) take a column of their engine's label plane when one is free
+ * (admission by label memory) and join the next iteration of the traversal already running
+ * (continuous batching); the column is given back at convergence, the answer kept in a result
+ * buffer until it is read.  Other queries run on `lanes` concurrent single-query lanes whose
+ * launches overlap on the device.  Every answer equals eeg_run's for the same query.  The
+ * graph must outlive the server; eeg_run calls on the same graph from other threads are
+ * allowed. */
 typedef struct eeg_server_cfg {
     uint64_t label_budget_bytes; /* HBM for the engines' label planes (0 = min(free/4, 48 GiB));
                                     split evenly over the BFS, SSSP and SSWP engines, each
@@ -259,12 +260,19 @@ typedef struct eeg_server_cfg {
     uint32_t K;                  /* virtual-node degree bound (0 = 8) */
     float damping;               /* PPR damping (0 = 0.85) */
     double ppr_epsilon;          /* PPR stop: L1(x_k+1 - x_k) < ppr_epsilon (0 = 1e-7) */
-    uint32_t flags;              /* EEG_SERVER_DISCARD_LABELS */
-    uint32_t reserved[7];
+    uint32_t flags;              /* EEG_SERVER_* below */
+    uint32_t lane_wgs_per_cu;    /* resident workgroups per CU of a single-query lane (0 = 4),
+                                    so its kernels leave room for other lanes and the engines */
+    uint32_t lanes;              /* single-query lanes (0 = 4, <= 64) */
+    uint32_t reserved[5];
 } eeg_server_cfg;
 /* Answers are not kept: columns are given back as soon as a query converges and wait() reports
  * status and times only (labels_out must be NULL).  For throughput runs. */
 #define EEG_SERVER_DISCARD_LABELS 1u
+/* SSSP and SSWP go through label-plane engines (continuous batching) instead of lanes. */
+#define EEG_SERVER_PLANE_ENGINES 2u
+/* BFS goes through lanes instead of the multi-source BFS engine. */
+#define EEG_SERVER_NO_BFS_ENGINE 4u
 typedef struct eeg_query_result {
     uint32_t algo;
     uint32_t source;
