@@ -50,6 +50,12 @@ def parse():
                    help="seconds per algorithm for the CPU baseline (extrapolated past it)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-energy", action="store_true")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched and packed section")
+    p.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    p.add_argument("--batch-scale", type=int, default=24)
+    p.add_argument("--serve-scale", type=int, default=26)
+    p.add_argument("--serve-qps", type=float, default=100.0)
+    p.add_argument("--serve-seconds", type=float, default=3.0)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 512 MiB device-to-device copy kernel three times "
                         "(scale reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -132,6 +138,154 @@ def cpu_baseline(scale, ef, seed, K, algs, cap):
                        + (" (capped runs: lower bound on time)" if extrapolated else "")),
             "per_algorithm": parts, "host_cpu": platform.processor() or platform.machine(),
             "nproc": os.cpu_count()}
+
+
+def pct(xs, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(xs), q)), 3)
+
+
+def batched_section(eg, a):
+    """Many queries per traversal against one-at-a-time launches of the same queries.
+    Multi-source: RMAT scale 24 (EF 16, seed 4), 256 sources ~ U[0, N) with seed 4, BFS with
+    64-bit visit masks and SSSP/SSWP label planes.  Packed: 64 RMAT graphs, scale ~ U{12..20},
+    edge factor ~ U{8..32} (seed 5), one source each (max out-degree vertex), in one traversal.
+    Latency of a query: from the start of the pass to its answer (a batched answer is ready when
+    the batch ends; a one-at-a-time answer after the queries before it)."""
+    import numpy as np
+
+    res = {}
+    g = eg.Graph.rmat(a.batch_scale, 16, seed=4)
+    n = g.info()["n"]
+    srcs = np.random.default_rng(4).integers(0, n, 256).astype(np.uint32)
+    multi = {}
+    for alg in ("bfs", "sssp", "sswp"):
+        g.run(alg, int(srcs[0]), want_labels=False)
+        g.run_batch(alg, srcs[:64], want_labels=False)
+        lat = []
+        t = time.perf_counter()
+        for x in srcs:
+            g.run(alg, int(x), want_labels=False)
+            lat.append((time.perf_counter() - t) * 1e3)
+        one = time.perf_counter() - t
+        t = time.perf_counter()
+        _, st = g.run_batch(alg, srcs, want_labels=False)
+        bat = time.perf_counter() - t
+        multi[alg] = {"one_at_a_time_qps": round(256 / one, 1), "batched_qps": round(256 / bat, 1),
+                      "speedup": round(one / bat, 2), "one_at_a_time_p99_ms": pct(lat, 99),
+                      "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
+        log(f"batched {alg}: {multi[alg]}")
+    res[f"rmat{a.batch_scale}_256_sources"] = multi
+    g.close()
+
+    rng = np.random.default_rng(5)
+    gs = [eg.Graph.rmat(int(rng.integers(12, 21)), int(rng.integers(8, 33)), seed=5000 + i) for i in range(64)]
+    psrc = np.array([x.info()["argmax_out_degree"] for x in gs], np.uint32)
+    packed = {"graphs": 64, "edges": int(sum(x.info()["m"] for x in gs))}
+    for alg in ("bfs", "sssp", "sswp", "cc"):
+        for x, y in zip(gs, psrc):
+            x.run(alg, int(y), want_labels=False)
+        eg.run_packed(gs, alg, psrc, want_labels=False)
+        lat = []
+        t = time.perf_counter()
+        for x, y in zip(gs, psrc):
+            x.run(alg, int(y), want_labels=False)
+            lat.append((time.perf_counter() - t) * 1e3)
+        one = time.perf_counter() - t
+        t = time.perf_counter()
+        _, st = eg.run_packed(gs, alg, psrc, want_labels=False)
+        bat = time.perf_counter() - t
+        packed[alg] = {"one_at_a_time_qps": round(64 / one, 1), "batched_qps": round(64 / bat, 1),
+                       "speedup": round(one / bat, 2), "one_at_a_time_p99_ms": pct(lat, 99),
+                       "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
+        log(f"packed {alg}: {packed[alg]}")
+    res["packed_64_rmat_graphs"] = packed
+    for x in gs:
+        x.close()
+    return res
+
+
+def serving_section(eg, a):
+    """Serving loop on RMAT scale 26 (EF 16, seed 6, w ~ U{1..64}): Poisson arrivals at
+    --serve-qps for --serve-seconds, mix 40% BFS / 30% SSSP / 20% SSWP / 10% PPR (damping 0.85,
+    one-hot restart), sources ~ U[0, N).  Server: BFS engine (continuous batching), other
+    queries on 2 concurrent single-query lanes, answers not kept.  Baseline: the same queries
+    run one at a time (eeg_run each, serially); its latencies are those of a FIFO queue with
+    the measured service times on the same arrival times.  Latency = answer time - arrival."""
+    import threading
+    import numpy as np
+
+    t = time.perf_counter()
+    g = eg.Graph.rmat(a.serve_scale, 16, seed=6)
+    n = g.info()["n"]
+    g.prepare(8, 1 | 4)  # Tigr of the out- and in-edge CSRs (PPR gathers over in-edges)
+    log(f"serving graph n={n} m={g.info()['m']} built in {time.perf_counter() - t:.2f} s")
+    rng = np.random.default_rng(6)
+    nq = max(1, int(a.serve_qps * a.serve_seconds))
+    mix = rng.choice(["bfs", "sssp", "sswp", "ppr"], size=nq, p=[0.4, 0.3, 0.2, 0.1])
+    srcs = rng.integers(0, n, nq)
+    gaps = rng.exponential(1.0 / a.serve_qps, nq)
+    arrive = np.cumsum(gaps)
+    srv = eg.Server(g, discard_labels=True)
+    for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
+        srv.wait(srv.submit(alg, 0))
+    tickets = [None] * nq
+    e0 = eg.energy_read(0)
+
+    def producer():
+        t0 = time.perf_counter()
+        for i in range(nq):
+            d = arrive[i] - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(mix[i], int(srcs[i]))
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, per, first, last, tprint = [], {}, None, 0.0, time.time()
+    for i in range(nq):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        _, r = srv.wait(tickets[i])
+        lat.append(r["t_done_ms"] - r["t_submit_ms"])
+        per.setdefault(mix[i], []).append(lat[-1])
+        first = r["t_submit_ms"] if first is None else min(first, r["t_submit_ms"])
+        last = max(last, r["t_done_ms"])
+        if time.time() - tprint > 5:
+            tprint = time.time()
+            log(f"serving: {i + 1}/{nq} answered")
+    th.join()
+    e1 = eg.energy_read(0)
+    served_s = (last - first) / 1e3
+    srv.close()
+    log("served stream done; one-at-a-time run of the same queries ...")
+    o = eg.make_opts(variant="sync_push_dd", K=8)
+    svc = []
+    e2 = eg.energy_read(0)
+    for i in range(nq):
+        t = time.perf_counter()
+        g.run(mix[i], int(srcs[i]), opts=o, want_labels=False)
+        svc.append(time.perf_counter() - t)
+    e3 = eg.energy_read(0)
+    fin, lat1 = 0.0, []
+    for i in range(nq):
+        fin = max(arrive[i], fin) + svc[i]
+        lat1.append((fin - arrive[i]) * 1e3)
+    one_s = fin - arrive[0] + svc[0]
+    g.close()
+    out = {"workload": f"rmat{a.serve_scale}_ef16_seed6, Poisson {a.serve_qps:g} q/s x {a.serve_seconds:g} s, "
+                       f"40/30/20/10 bfs/sssp/sswp/ppr",
+           "queries": nq,
+           "served": {"qps": round(nq / served_s, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                      "per_algorithm_p50_p99_ms": {str(k): [pct(v, 50), pct(v, 99)] for k, v in per.items()},
+                      "j_per_query": round((e1 - e0) * 1e-6 / nq, 4) if e0 is not None else None},
+           "one_at_a_time": {"qps": round(nq / one_s, 1), "capacity_qps": round(nq / sum(svc), 1),
+                             "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99),
+                             "j_per_query": round((e3 - e2) * 1e-6 / nq, 4) if e2 is not None else None}}
+    out["one_at_a_time"]["qps"] = float(out["one_at_a_time"]["qps"])
+    out["qps_ratio"] = round(float(out["served"]["qps"] / out["one_at_a_time"]["qps"]), 2)
+    log(f"serving: {out}")
+    return out
 
 
 def main():
@@ -241,6 +395,9 @@ def main():
     if not a.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(a.scale, a.edge_factor, a.seed, a.K, algs, a.cpu_cap)
+    g.close()
+    batched = None if a.no_batch else batched_section(eg, a)
+    serving = None if a.no_serve else serving_section(eg, a)
 
     out = {
         "metric": METRIC,
@@ -263,9 +420,10 @@ def main():
         "cpu_baseline": cpu,
         "per_algorithm": per_alg,
         "energy": energy,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
-    g.close()
 
 
 if __name__ == "__main__":

@@ -224,27 +224,27 @@ class Graph:
                                    C.byref(o), C.byref(ms)), f"eeg_autotune({algo})")
         return o, ms.value
 
-    def run_batch(self, algo, sources, opts=None, **kw):
+    def run_batch(self, algo, sources, opts=None, want_labels=True, **kw):
+        """Many sources in one traversal; returns (labels[nsrc, n] or None, stats dict)."""
         a = L.ALGOS[algo] if isinstance(algo, str) else int(algo)
         o = opts if opts is not None else make_opts(**kw)
         srcs = _u32(np.asarray(sources))
         n = self.info()["n"]
-        out = np.empty((len(srcs), n), np.float32 if a in (L.PR, L.PPR) else np.uint32)
+        out = np.empty((len(srcs), n), np.uint32) if want_labels else None
         st = L.Stats()
         L.check(L.lib.eeg_run_batch(self._h, a, _ptr(srcs), len(srcs), C.byref(o), _ptr(out),
                                     C.byref(st)), f"eeg_run_batch({algo})")
         return out, st.to_dict()
 
 
-def run_packed(graphs, algo, sources, opts=None, **kw):
-    """Several graphs in one launch sequence; returns a list of label arrays."""
+def run_packed(graphs, algo, sources, opts=None, want_labels=True, **kw):
+    """Several graphs in one launch sequence; returns (list of label arrays or None, stats)."""
     a = L.ALGOS[algo] if isinstance(algo, str) else int(algo)
     o = opts if opts is not None else make_opts(**kw)
     hs = (C.c_void_p * len(graphs))(*[g.handle.value for g in graphs])
     srcs = _u32(np.asarray(sources))
-    outs = [np.empty(g.info()["n"], np.float32 if a in (L.PR, L.PPR) else np.uint32)
-            for g in graphs]
-    ptrs = (C.c_void_p * len(graphs))(*[o_.ctypes.data for o_ in outs])
+    outs = [np.empty(g.info()["n"], np.uint32) for g in graphs] if want_labels else None
+    ptrs = (C.c_void_p * len(graphs))(*[o_.ctypes.data for o_ in outs]) if want_labels else None
     st = L.Stats()
     L.check(L.lib.eeg_run_packed(hs, len(graphs), a, _ptr(srcs), C.byref(o), ptrs, C.byref(st)),
             "eeg_run_packed")

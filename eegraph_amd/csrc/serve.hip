@@ -548,8 +548,8 @@ void lane_loop(eeg_server* srv, Lane* ln) {
         memset(&o, 0, sizeof(o));
         o.variant = EEG_SYNC_PUSH_DD;
         o.K = srv->K;
-        // a small resident grid leaves room on every CU for the other lanes and the engines
-        o.wgs_per_cu = srv->cfg.lane_wgs_per_cu ? srv->cfg.lane_wgs_per_cu : 4;
+        // a partial resident grid leaves room on every CU for the other lanes and the engines
+        o.wgs_per_cu = srv->cfg.lane_wgs_per_cu ? srv->cfg.lane_wgs_per_cu : 16;
         o.damping = srv->cfg.damping;
         o.epsilon = srv->cfg.ppr_epsilon;
         const uint32_t a = tk->algo;
@@ -646,7 +646,7 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     }
     EEG_HIP(hipStreamCreateWithFlags(&srv->io_st, hipStreamNonBlocking));
     EEG_TRY(srv->io_buf.alloc((size_t)n * 4 + 4));
-    const uint32_t nl = c.lanes ? c.lanes : 4;
+    const uint32_t nl = c.lanes ? c.lanes : 2;
     for (uint32_t i = 0; i < nl; ++i) {
         srv->lanes.emplace_back(new (std::nothrow) Lane());
         if (!srv->lanes.back()) return fail(EEG_ERR_OOM, "host allocation failed");

@@ -261,9 +261,9 @@ typedef struct eeg_server_cfg {
     float damping;               /* PPR damping (0 = 0.85) */
     double ppr_epsilon;          /* PPR stop: L1(x_k+1 - x_k) < ppr_epsilon (0 = 1e-7) */
     uint32_t flags;              /* EEG_SERVER_* below */
-    uint32_t lane_wgs_per_cu;    /* resident workgroups per CU of a single-query lane (0 = 4),
+    uint32_t lane_wgs_per_cu;    /* resident workgroups per CU of a single-query lane (0 = 16),
                                     so its kernels leave room for other lanes and the engines */
-    uint32_t lanes;              /* single-query lanes (0 = 4, <= 64) */
+    uint32_t lanes;              /* single-query lanes (0 = 2, <= 64) */
     uint32_t reserved[5];
 } eeg_server_cfg;
 /* Answers are not kept: columns are given back as soon as a query converges and wait() reports
