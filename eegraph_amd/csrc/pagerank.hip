@@ -107,11 +107,15 @@ __device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Com
     res += fabs(((double)hi + (double)lo) - xo);
 }
 
+// Edge slots per pull tile: twice the traversal tile, so each lane keeps 16 independent
+// gathers in flight and the per-tile staging is amortised over more edges.
+constexpr int PR_TILE = 2 * TILE_EDGES;
+
 template <int K>
 __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
     constexpr int VPR = WG / K;
-    constexpr int TV = TILE_EDGES / K;
-    constexpr int ROUNDS = TILE_EDGES / WG;
+    constexpr int TV = PR_TILE / K;
+    constexpr int ROUNDS = PR_TILE / WG;
     constexpr int EMAX = TV + 2;
     __shared__ uint32_t sS[EMAX];
     __shared__ uint64_t sRow[EMAX];
@@ -359,7 +363,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     EEG_TRY(ensure_in(g));
     EEG_TRY(ensure_tigr(g, g->in, g->tin, K));
     const Tigr& t = g->tin[k_index(K)];
-    const uint32_t TV = TILE_EDGES / K;
+    const uint32_t TV = PR_TILE / K;
     const uint64_t ntiles = (t.P + TV - 1) / TV;
 
     EEG_TRY(w.ctl.alloc(sizeof(Ctl)));
