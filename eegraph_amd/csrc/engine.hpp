@@ -78,6 +78,9 @@ struct Tigr {
     DevMem vscan;  // u32[nz+1]
     DevMem vnc;    // u32[n]: ceil(deg/K) per vertex (virtual nodes a frontier entry brings)
     bool built = false;
+    DevMem span;        // in-edge tables only: positions of vertices crossing a PageRank tile
+    uint32_t nspan = 0;
+    bool span_built = false;
 };
 
 // Reusable per-graph traversal state (one query at a time per handle).
@@ -148,6 +151,7 @@ eeg_status build_csr_from_device_edges(eeg_graph* g, const uint32_t* d_src, cons
 eeg_status ensure_tigr(eeg_graph* g, Csr& c, Tigr* table, uint32_t K);
 eeg_status ensure_sym(eeg_graph* g);
 eeg_status ensure_in(eeg_graph* g);
+eeg_status ensure_pr_span(eeg_graph* g, uint32_t K);  // pagerank.hip, after ensure_tigr(in)
 eeg_status export_tigr(eeg_graph* g, uint32_t K, uint64_t* node_pointer, uint32_t* edge_list,
                        uint32_t* parts, uint64_t* num_parts);
 eeg_status orig_csr(eeg_graph* g, const Csr& src, Csr& out);

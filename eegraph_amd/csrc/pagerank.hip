@@ -187,7 +187,8 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                     const uint32_t eoff = (t - sS[k]) * K + lj;
                     if (eoff < sDeg[k]) {
                         pv[r] = true;
-                        pidx[r] = a.col[sRow[k] + eoff];
+                        // streamed once per iteration: keep it from evicting the gathered contributions
+                        pidx[r] = __builtin_nontemporal_load(&a.col[sRow[k] + eoff]);
                     }
                 }
             }
@@ -343,6 +344,32 @@ __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz,
 
 }  // namespace
 
+// Vertices of the in-edge Tigr table whose virtual nodes cross a pull tile, listed once per
+// graph and K (their partial sums are combined by k_pr_fix).  Called with the graph's lock held
+// or before concurrent queries start.
+eeg_status ensure_pr_span(eeg_graph* g, uint32_t K) {
+    Tigr& t = g->tin[k_index(K)];
+    if (t.span_built) return EEG_OK;
+    const uint32_t TV = PR_TILE / K;
+    const uint64_t ntiles = (t.P + TV - 1) / TV;
+    hipStream_t s = g->stream;
+    DevMem cnt;
+    EEG_TRY(t.span.alloc((size_t)ntiles * 4 + 4));
+    EEG_TRY(cnt.alloc(4));
+    EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    if (t.nz) {
+        k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
+            t.vscan.as<uint32_t>(), t.nz, TV, t.span.as<uint32_t>(), cnt.as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+    }
+    uint32_t nspan = 0;
+    EEG_HIP(hipMemcpyAsync(&nspan, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    EEG_HIP(hipStreamSynchronize(s));
+    t.nspan = nspan;
+    t.span_built = true;
+    return EEG_OK;
+}
+
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st, Work* wk, hipStream_t sk) {
     const auto t_host0 = std::chrono::steady_clock::now();
@@ -375,18 +402,9 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     }
     EEG_TRY(w.spill.alloc((size_t)(ntiles + 1) * 2 * sizeof(float2)));
     // vertices whose virtual nodes cross a tile boundary (finished by k_pr_fix)
-    DevMem span, cnt;
-    EEG_TRY(span.alloc((size_t)ntiles * 4 + 4));
-    EEG_TRY(cnt.alloc(4));
-    EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
-    if (t.nz) {
-        k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
-            t.vscan.as<uint32_t>(), t.nz, TV, span.as<uint32_t>(), cnt.as<uint32_t>());
-        EEG_HIP(hipGetLastError());
-    }
-    uint32_t nspan = 0;
-    EEG_HIP(hipMemcpyAsync(&nspan, cnt.p, 4, hipMemcpyDeviceToHost, s));
-    EEG_HIP(hipStreamSynchronize(s));
+    EEG_TRY(ensure_pr_span(g, K));
+    const DevMem& span = t.span;
+    const uint32_t nspan = t.nspan;
     // the fix-up order only matters within a vertex, so the list order is irrelevant
 
     int cus = 256;

@@ -181,8 +181,9 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
                     if (act && eoff < deg) {
                         const uint64_t e = sRow[k] + eoff;
                         ev[r] = true;
-                        ed[r] = a.col[e];
-                        if constexpr (R::weighted) ew[r] = a.wgt[e];
+                        // edge data is read once per visit: keep it from evicting labels
+                        ed[r] = __builtin_nontemporal_load(&a.col[e]);
+                        if constexpr (R::weighted) ew[r] = __builtin_nontemporal_load(&a.wgt[e]);
                         el[r] = lab;
                         ++st_edge;
                     }

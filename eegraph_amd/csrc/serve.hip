@@ -518,7 +518,8 @@ eeg_status prepare_for(eeg_server* srv, uint32_t algo) {
     }
     if (algo == EEG_PR || algo == EEG_PPR) {
         EEG_TRY(ensure_in(g));
-        return ensure_tigr(g, g->in, g->tin, srv->K);
+        EEG_TRY(ensure_tigr(g, g->in, g->tin, srv->K));
+        return ensure_pr_span(g, srv->K);
     }
     return ensure_tigr(g, g->out, g->tout, srv->K);
 }
