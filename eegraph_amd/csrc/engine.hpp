@@ -114,6 +114,7 @@ struct eeg_graph {
     eeg::Tigr tout[4], tsym[4], tin[4];  // index log2(K) - 2
     eeg::Work work;
     eeg::DevMem perm, inv;           // degree order: perm[original] = internal, inv = perm^-1
+    eeg::DevMem outdeg;              // u32 out-degrees (built with the PageRank tables)
     std::vector<uint32_t> h_perm;    // host copy of perm (sources map without a device read)
     uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
     uint64_t serial = 0;             // unique per handle in this process

@@ -72,7 +72,7 @@ struct PullArgs {
     float* x_new;
     float* xl_new;
     float* c_new;
-    const uint64_t* out_row; // out-degrees for the new contributions
+    const uint32_t* outdeg;  // out-degrees for the new contributions
     float2* spill;           // per tile: head and tail partials
     double* res_part;        // per workgroup FP64 residual
     Ctl* ctl;
@@ -96,13 +96,13 @@ __device__ __forceinline__ void rank_pair(float base, float d, Comp sum, float* 
     *lo = small - (h - q.s);
 }
 
-__device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Comp sum, double& res) {
+// New rank of v from its gathered sum; xo = old rank (hi + lo), d = out-degree.
+__device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Comp sum, double xo, uint32_t d,
+                                              double& res) {
     float hi, lo;
     rank_pair(v == a.pers ? a.pers_base : a.base, a.damping, sum, &hi, &lo);
-    const double xo = (double)a.x_old[v] + (double)a.xl_old[v];
     a.x_new[v] = hi;
     a.xl_new[v] = lo;
-    const uint64_t d = a.out_row[v + 1] - a.out_row[v];
     a.c_new[v] = d ? (hi + lo) / (float)d : 0.0f;
     res += fabs(((double)hi + (double)lo) - xo);
 }
@@ -122,6 +122,8 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
     __shared__ uint32_t sDeg[EMAX];
     __shared__ uint16_t sOwn[TV];
     __shared__ float2 sPart[TV];
+    __shared__ float2 sXo[EMAX];     // old rank (hi, lo) of each entry, loaded with its row
+    __shared__ uint32_t sOd[EMAX];   // out-degree of each entry
     __shared__ uint32_t sMisc[4];
     __shared__ double sRed[WG / WAVE];
 
@@ -155,11 +157,15 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
             __syncthreads();
             const int nent = (int)sMisc[0];
             const int nnext = (int)sMisc[1];
+            // entry data, including what finishing the vertex needs, so those loads overlap
+            // with the row loads instead of trailing the gathers
             for (int k = threadIdx.x; k < nent; k += WG) {
                 const uint32_t v = a.vlist[e_start + k];
                 const uint64_t r0 = a.row[v];
                 sRow[k] = r0;
                 sDeg[k] = (uint32_t)(a.row[v + 1] - r0);
+                sXo[k] = make_float2(a.x_old[v], a.xl_old[v]);
+                sOd[k] = a.outdeg[v];
             }
             for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
                 const uint32_t t = t0 + j;
@@ -213,7 +219,8 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                 const bool started_before = sS[k] < t0;
                 const bool continues = sS[k + 1] > t1;
                 if (!started_before && !continues) {
-                    finish_vertex(a, a.vlist[e_start + k], acc, res);
+                    const float2 xo = sXo[k];
+                    finish_vertex(a, a.vlist[e_start + k], acc, (double)xo.x + (double)xo.y, sOd[k], res);
                 } else if (started_before) {
                     a.spill[2 * tile] = make_float2(acc.s, acc.e);      // head
                 } else {
@@ -252,7 +259,8 @@ __global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __res
             const float2 h = a.spill[2 * t];
             acc = comp_add(acc, Comp{h.x, h.y});
         }
-        finish_vertex(a, a.vlist[pos], acc, res);
+        const uint32_t v = a.vlist[pos];
+        finish_vertex(a, v, acc, (double)a.x_old[v] + (double)a.xl_old[v], a.outdeg[v], res);
     }
     res = wave_sum(res);
     if (lane_id() == 0) sRed[threadIdx.x / WAVE] = res;
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __res
 }
 
 // Vertices without in-edges hold their teleport term from the first iteration on.
-__global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint64_t* __restrict__ out_row,
+__global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint32_t* __restrict__ outdeg,
                               uint32_t n, const float* __restrict__ x_old, const float* __restrict__ xl_old,
                               float* __restrict__ x_new, float* __restrict__ xl_new,
                               float* __restrict__ c_new, float base0, uint32_t pers, float pers_base,
@@ -279,7 +287,7 @@ __global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint64_
         const float base = (v == pers) ? pers_base : base0;
         x_new[v] = base;
         xl_new[v] = 0.0f;
-        const uint64_t d = out_row[v + 1] - out_row[v];
+        const uint32_t d = outdeg[v];
         c_new[v] = d ? base / (float)d : 0.0f;
         res += fabs((double)base - ((double)x_old[v] + (double)xl_old[v]));
     }
@@ -314,14 +322,14 @@ __global__ void k_pr_reduce(const double* __restrict__ part, uint32_t nparts, Ct
 }
 
 // x0 = 1/N everywhere, or e_s when personalised (pers = s)
-__global__ void k_pr_init(const uint64_t* __restrict__ out_row, uint32_t n, float x0, uint32_t pers,
+__global__ void k_pr_init(const uint32_t* __restrict__ outdeg, uint32_t n, float x0, uint32_t pers,
                           float* __restrict__ x, float* __restrict__ xl, float* __restrict__ c) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         const float xv = (pers == 0xffffffffu) ? x0 : (v == pers ? 1.0f : 0.0f);
         x[v] = xv;
         xl[v] = 0.0f;
-        const uint64_t d = out_row[v + 1] - out_row[v];
+        const uint32_t d = outdeg[v];
         c[v] = d ? xv / (float)d : 0.0f;
     }
 }
@@ -331,6 +339,12 @@ __global__ void k_pr_output(const float* __restrict__ hi, const float* __restric
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x)
         out[v] = hi[v] + lo[v];
+}
+
+__global__ void k_outdeg(const uint64_t* __restrict__ row, uint32_t n, uint32_t* __restrict__ deg) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        deg[v] = (uint32_t)(row[v + 1] - row[v]);
 }
 
 __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV,
@@ -350,6 +364,14 @@ __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz,
 eeg_status ensure_pr_span(eeg_graph* g, uint32_t K) {
     Tigr& t = g->tin[k_index(K)];
     if (t.span_built) return EEG_OK;
+    if (!g->outdeg.p) {  // out-degrees as u32, read by every PageRank iteration
+        EEG_TRY(g->outdeg.alloc((size_t)g->n * 4 + 4));
+        if (g->n) {
+            k_outdeg<<<std::max(1u, std::min(8192u, (g->n + WG - 1) / WG)), WG, 0, g->stream>>>(
+                g->out.rowp(), g->n, g->outdeg.as<uint32_t>());
+            EEG_HIP(hipGetLastError());
+        }
+    }
     const uint32_t TV = PR_TILE / K;
     const uint64_t ntiles = (t.P + TV - 1) / TV;
     hipStream_t s = g->stream;
@@ -422,7 +444,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     const float x0 = 1.0f / (float)n;
     const float base = personalised ? 0.0f : (1.0f - d) / (float)n;
     const uint32_t pers = personalised ? source : 0xffffffffu;
-    k_pr_init<<<grid_iso, WG, 0, s>>>(g->out.rowp(), n, x0, pers, w.fx[0].as<float>(), w.fl[0].as<float>(),
+    k_pr_init<<<grid_iso, WG, 0, s>>>(g->outdeg.as<uint32_t>(), n, x0, pers, w.fx[0].as<float>(), w.fl[0].as<float>(),
                                       w.fc[0].as<float>());
     EEG_HIP(hipGetLastError());
 
@@ -433,7 +455,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     pa.vscan = t.vscan.as<uint32_t>();
     pa.nz = t.nz;
     pa.P = (uint32_t)t.P;
-    pa.out_row = g->out.rowp();
+    pa.outdeg = g->outdeg.as<uint32_t>();
     pa.spill = w.spill.as<float2>();
     pa.ctl = ctl;
     pa.base = base;
@@ -472,7 +494,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         }
         const uint32_t iso_iters = inplace ? 1 : 2;  // vertices without in-edges settle at once
         if (it <= iso_iters) {
-            k_pr_isolated<<<grid_iso, WG, 0, s>>>(g->in.rowp(), g->out.rowp(), n, pa.x_old, pa.xl_old,
+            k_pr_isolated<<<grid_iso, WG, 0, s>>>(g->in.rowp(), g->outdeg.as<uint32_t>(), n, pa.x_old, pa.xl_old,
                                                   pa.x_new, pa.xl_new, pa.c_new, base, pers, 1.0f - d,
                                                   red + grid_pull + grid_fix, ctl);
             EEG_HIP(hipGetLastError());
