@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--no-energy", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the batched and packed section")
     p.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    p.add_argument("--no-grid", action="store_true", help="skip the grid section")
     p.add_argument("--batch-scale", type=int, default=24)
     p.add_argument("--serve-scale", type=int, default=26)
     p.add_argument("--serve-qps", type=float, default=100.0)
@@ -202,6 +203,32 @@ def batched_section(eg, a):
     res["packed_64_rmat_graphs"] = packed
     for x in gs:
         x.close()
+    return res
+
+
+def grid_section(eg, a):
+    """High-diameter case: 4096 x 4096 grid, 4-neighbour edges both ways plus 1% random
+    shortcut pairs (seed 3), w ~ U{1..1000}, source 0.  BFS/SSSP/SSWP/CC with compacted
+    frontier queues against flag scans (sync_push_dd), and the async variants; device
+    traversal time of one query each (after one untimed run)."""
+    t = time.perf_counter()
+    s, d, w = eg.gen_grid(4096, 4096, 0.01, seed=3, wmin=1, wmax=1000)
+    g = eg.Graph.from_edges(s, d, w, n=4096 * 4096)
+    del s, d, w
+    m = g.info()["m"]
+    log(f"grid graph m={m} built in {time.perf_counter() - t:.2f} s")
+    res = {"workload": "grid4096x4096 + 1% shortcuts, seed 3, w 1..1000, source 0", "edges": m}
+    for alg in ("bfs", "sssp", "sswp", "cc"):
+        r = {}
+        for name, var, fr in (("sync_compact", "sync_push_dd", "compact"), ("sync_flags", "sync_push_dd", "flags"),
+                              ("async_push_dd", "async_push_dd", "compact"), ("async_push_td", "async_push_td", "compact")):
+            g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False)
+            _, st = g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False)
+            r[name] = {"ms": round(st["ms_traversal"], 3), "gteps": round(m / st["ms_traversal"] / 1e6, 2),
+                       "iterations": st["iterations"]}
+        res[alg] = r
+        log(f"grid {alg}: {r}")
+    g.close()
     return res
 
 
@@ -396,6 +423,7 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(a.scale, a.edge_factor, a.seed, a.K, algs, a.cpu_cap)
     g.close()
+    grid = None if a.no_grid else grid_section(eg, a)
     batched = None if a.no_batch else batched_section(eg, a)
     serving = None if a.no_serve else serving_section(eg, a)
 
@@ -420,6 +448,7 @@ def main():
         "cpu_baseline": cpu,
         "per_algorithm": per_alg,
         "energy": energy,
+        "grid": grid,
         "batched": batched,
         "serving": serving,
     }

@@ -431,7 +431,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
 
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
-    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : 24;
+    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : 16;
     const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * wpc));
     const unsigned grid_fix = std::max(1u, std::min(1024u, (nspan + WG - 1) / WG));
     const unsigned grid_iso = std::max(1u, std::min(4096u, (n + WG - 1) / WG));
