@@ -96,6 +96,9 @@ struct Work {
     DevMem spill;       // PR: per-tile partial sums
     DevMem red;         // PR: per-workgroup residual partials
     uint32_t epoch = 1; // iteration stamps already used on stamp/flag
+    DevMem devit;       // replayed iteration chunks: {iteration, stamp base}
+    hipGraphExec_t graph_exec = nullptr;  // captured chunk and the arguments it was captured with
+    std::vector<uint8_t> graph_key;
     Ctl* host_ctl = nullptr;  // pinned mirrors of Ctl (two, for overlapped polling)
     std::vector<hipEvent_t> events;  // pool: one or two per launch
     eeg_status event(size_t i, hipEvent_t* out);

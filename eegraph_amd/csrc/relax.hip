@@ -20,6 +20,7 @@
 //           reserves queue slots and virtual-node prefix together.
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <vector>
 
 #include "engine.hpp"
@@ -51,11 +52,29 @@ struct RelaxArgs {
     uint32_t it;        // 1-based iteration of this traversal
     uint32_t stampv;    // stamp value of this iteration
     uint32_t count_stats;
+    // Replayed graph (M_QUEUE): when dev_it is set, dev_it[0] is the iteration and dev_it[1]
+    // the stamp base, read from device memory; it, stampv, the control slots and the queue
+    // ping-pong are derived from them, so one captured chunk serves every later chunk and run.
+    const uint32_t* dev_it;
+    uint32_t *qa, *sa, *qb, *sb;  // queue buffers 0 and 1
 };
 
 template <int ALG, int K, int MODE>
-__global__ __launch_bounds__(WG) void k_relax(RelaxArgs a) {
+__global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
     using R = Rule<ALG>;
+    RelaxArgs a = args;
+    if (MODE == M_QUEUE && a.dev_it) {
+        const uint32_t it = *a.dev_it;
+        const bool odd = ((it - 1) & 1) != 0;  // input queue = buffer (it-1)&1
+        a.it = it;
+        a.stampv = a.dev_it[1] + it;
+        a.qslot_in = (int)((it - 1) % 3);
+        a.qslot_out = (int)(it % 3);
+        a.qin = odd ? a.qb : a.qa;
+        a.sin = odd ? a.sb : a.sa;
+        a.qout = odd ? a.qa : a.qb;
+        a.sout = odd ? a.sa : a.sb;
+    }
     constexpr int VPR = WG / K;             // virtual nodes per round
     constexpr int TV = TILE_EDGES / K;      // virtual nodes per tile
     constexpr int ROUNDS = TILE_EDGES / WG;
@@ -322,6 +341,11 @@ __global__ void k_init_labels(uint32_t* __restrict__ lab, uint32_t n, uint32_t f
     }
 }
 
+// Advances the device iteration counter of a replayed chunk (one thread, vector store).
+__global__ void k_tick(uint32_t* it) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *it += 1;
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint64_t n, uint32_t val) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -456,6 +480,77 @@ size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
     if (mode == M_ASYNC) b += (size_t)P * 4;
     return b;
 }
+
+namespace {
+
+// Iterations per replayed chunk: a multiple of 2 (queue ping-pong) and 3 (control slots), so
+// every replay starts in the same state pattern.
+constexpr uint32_t GRAPH_CHUNK = 6;
+
+// The executable graph of one chunk for these kernel arguments: built on first use and kept
+// in the work state until the arguments (buffers, grid, kernel) change.  It holds kernel
+// nodes only.
+eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga, hipStream_t s,
+                       hipGraphExec_t* out) {
+    std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid));
+    memcpy(key.data(), &ga, sizeof(RelaxArgs));
+    memcpy(key.data() + sizeof(RelaxArgs), &kfn, sizeof(kfn));
+    memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn), &grid, sizeof(grid));
+    if (w.graph_exec && w.graph_key == key) {
+        *out = w.graph_exec;
+        return EEG_OK;
+    }
+    if (w.graph_exec) {
+        (void)hipGraphExecDestroy(w.graph_exec);
+        w.graph_exec = nullptr;
+    }
+    // built node by node (no stream capture, which would interact with other threads'
+    // HIP calls): relaxation -> tick -> relaxation -> ... in one dependency chain
+    hipGraph_t graph = nullptr;
+    EEG_HIP(hipGraphCreate(&graph, 0));
+    RelaxArgs a = ga;
+    uint32_t* itp = const_cast<uint32_t*>(ga.dev_it);
+    void* rargs[] = {&a};
+    void* targs[] = {&itp};
+    hipError_t e = hipSuccess;
+    hipGraphNode_t prev = nullptr;
+    for (uint32_t i = 0; i < GRAPH_CHUNK && e == hipSuccess; ++i) {
+        hipKernelNodeParams kp{};
+        kp.func = (void*)kfn;
+        kp.gridDim = dim3(grid);
+        kp.blockDim = dim3(WG);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = rargs;
+        kp.extra = nullptr;
+        hipGraphNode_t nr = nullptr, nt = nullptr;
+        e = hipGraphAddKernelNode(&nr, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+        if (e != hipSuccess) break;
+        hipKernelNodeParams tp{};
+        tp.func = (void*)k_tick;
+        tp.gridDim = dim3(1);
+        tp.blockDim = dim3(64);
+        tp.sharedMemBytes = 0;
+        tp.kernelParams = targs;
+        tp.extra = nullptr;
+        e = hipGraphAddKernelNode(&nt, graph, &nr, 1, &tp);
+        prev = nt;
+    }
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(EEG_ERR_DEVICE, "building the iteration chunk failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&w.graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        w.graph_exec = nullptr;
+        return fail(EEG_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    w.graph_key = key;
+    *out = w.graph_exec;
+    return EEG_OK;
+}
+
+}  // namespace
 
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                          eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds, Work* wk, hipStream_t sk) {
@@ -616,6 +711,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     EEG_HIP(hipEventRecord(ev_start, s));
     uint32_t it = 1;
     eeg_status rc = EEG_OK;
+    uint32_t graph_launches = 0;  // nonzero: the replayed-graph path ran
     std::vector<uint32_t> activity;
     if (o.collect_activity) {
         // one launch at a time so every frontier size can be read back
@@ -632,6 +728,64 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
                 break;
             }
         }
+    } else if (mode == M_QUEUE && !o.timing) {
+        // Iteration 1 eagerly (it may read the static entry list), then a graph of GRAPH_CHUNK
+        // iterations (relaxation + counter tick each) replayed until the control block reports
+        // done.  The next replay is queued before the previous one's control
+        // block is inspected.
+        EEG_TRY(launch(1));
+        it = 2;
+        EEG_TRY(w.devit.alloc(8));
+        uint32_t* dit = w.devit.as<uint32_t>();
+        k_fill_u32<<<1, 64, 0, s>>>(dit, 1, 2u);
+        k_fill_u32<<<1, 64, 0, s>>>(dit + 1, 1, base);
+        EEG_HIP(hipGetLastError());
+        RelaxArgs ga = ra;
+        ga.it = 0;
+        ga.stampv = 0;
+        ga.qslot_in = 0;
+        ga.qslot_out = 0;
+        ga.qin = ga.sin = nullptr;
+        ga.qout = ga.sout = nullptr;
+        ga.dev_it = dit;
+        ga.qa = w.q[0].as<uint32_t>();
+        ga.sa = w.s[0].as<uint32_t>();
+        ga.qb = w.q[1].as<uint32_t>();
+        ga.sb = w.s[1].as<uint32_t>();
+        hipGraphExec_t exec = nullptr;
+        EEG_TRY(chunk_graph(w, kfn, grid_queue, ga, s, &exec));
+        hipEvent_t poll[2];
+        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
+        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        uint32_t replays = 0;
+        auto submit = [&](int slot) -> eeg_status {
+            EEG_HIP(hipGraphLaunch(exec, s));
+            ++replays;
+            it += GRAPH_CHUNK;
+            hipEvent_t er;
+            EEG_TRY(w.event(2 + replays, &er));
+            EEG_HIP(hipEventRecord(er, s));
+            EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            EEG_HIP(hipEventRecord(poll[slot], s));
+            return EEG_OK;
+        };
+        int cur = 0;
+        rc = submit(0);
+        while (rc == EEG_OK) {
+            if (it <= max_iters + GRAPH_CHUNK) rc = submit(cur ^ 1);
+            if (rc != EEG_OK) break;
+            EEG_HIP(hipEventSynchronize(poll[cur]));
+            const Ctl& h = w.host_ctl[cur];
+            if (h.err || h.done) break;
+            if (it > max_iters + GRAPH_CHUNK) {
+                rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
+                break;
+            }
+            cur ^= 1;
+        }
+        (void)hipEventDestroy(poll[0]);
+        (void)hipEventDestroy(poll[1]);
+        graph_launches = 1 + replays * GRAPH_CHUNK;
     } else {
         // Chunks of launches; the next chunk is queued before the previous chunk's control
         // block is inspected, so the GPU never waits for the host between chunks.
@@ -671,12 +825,16 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     w.epoch = base + it + 2;
 
     if (st) {
+        // end of the last working iteration: its launch, or the replay that contains it
         hipEvent_t ev_end;
-        EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(hc.iters, 1), &ev_end));
+        if (graph_launches && hc.iters > 1)
+            EEG_TRY(w.event(2 + (hc.iters - 1 + GRAPH_CHUNK - 1) / GRAPH_CHUNK, &ev_end));
+        else
+            EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(hc.iters, 1), &ev_end));
         float ms = 0;
         EEG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end));
         st->iterations = hc.iters;
-        st->launches = it - 1;
+        st->launches = graph_launches ? graph_launches : it - 1;
         st->ms_traversal = ms;
         st->edges_relaxed = hc.stat[2];
         st->vnodes_processed = hc.stat[1];

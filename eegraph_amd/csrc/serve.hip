@@ -38,6 +38,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -72,6 +73,7 @@ struct Ticket {
     bool cancel = false;
     double t_submit = 0, t_start = 0, t_done = 0;
     std::vector<uint32_t> host;  // answer in host memory (lane, or result pool exhausted)
+    std::string err;             // why the query failed
     DevMem* res = nullptr;       // answer in a pooled device buffer
     std::shared_ptr<EvHold> ev;  // completes when `res` holds the answer
 };
@@ -572,6 +574,7 @@ void lane_loop(eeg_server* srv, Lane* ln) {
         }
         std::lock_guard<std::mutex> lk(srv->mu);
         tk->status = rc;
+        if (rc != EEG_OK) tk->err = eeg_last_error();
         tk->state = T_DONE;
         tk->t_done = srv->now_ms();
         tk->host.swap(out);
@@ -724,7 +727,7 @@ extern "C" eeg_status eeg_server_wait(eeg_server* s, uint64_t ticket, void* labe
     tk->res = nullptr;
     if (tk->status != EEG_OK) {
         if (buf && e) e->res_free.push_back(buf);
-        return fail((eeg_status)tk->status, "query failed");
+        return fail((eeg_status)tk->status, "query failed: %s", tk->err.empty() ? "device error" : tk->err.c_str());
     }
     if (!buf) {  // answer in host memory (or labels discarded)
         if (labels_out && !tk->host.empty()) memcpy(labels_out, tk->host.data(), tk->host.size() * 4);

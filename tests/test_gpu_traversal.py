@@ -167,3 +167,21 @@ def test_symmetrised_and_in_edge_csr(rmat14):
     irow, icol, _ = O.csr(n, d, s)
     assert np.array_equal(row, irow)
     assert np.array_equal(_rows_sorted(row, col), _rows_sorted(irow, icol))
+
+
+@pytest.mark.parametrize("alg", ["bfs", "sssp", "sswp", "cc"])
+def test_replayed_chunks_match_eager_launches(rmat14, alg):
+    """sync_push_dd with compact frontiers replays a captured chunk of iterations unless
+    per-launch timing is asked for.  The two paths alternate, with other queries in between,
+    and every replayed run must equal the eager run and the oracle."""
+    n, s, d, w, g = rmat14
+    rng = np.random.default_rng(17)
+    ref_alg = {"bfs": O.BFS, "sssp": O.SSSP, "sswp": O.SSWP, "cc": O.CC}[alg]
+    for rep in range(4):
+        src = int(rng.integers(0, n)) if rep else g.info()["argmax_out_degree"]
+        replayed, st_r = g.run(alg, src, variant="sync_push_dd", K=8)
+        g.run("bfs", (src + 1) % n, variant="async_push_dd", K=8)  # other work in between
+        eager, st_e = g.run(alg, src, variant="sync_push_dd", K=8, timing=True)
+        assert np.array_equal(replayed, eager), (alg, rep)
+        assert st_r["iterations"] == st_e["iterations"]
+        assert np.array_equal(replayed, O.reference_labels(ref_alg, n, s, d, w, src)), (alg, rep)
