@@ -31,9 +31,36 @@ unsigned grid_n(uint64_t work, unsigned cap = 16384) {
 }
 
 
+// Replayed graphs: the per-iteration arguments come from device memory (see MsArgs).
+__device__ __forceinline__ void ms_resolve(MsArgs& a, bool finalize) {
+    if (!a.dev_it) return;
+    const uint32_t lvl = a.dev_it[0];
+    const bool odd = ((lvl - 1) & 1) != 0;  // input frontier = buffer (level-1)&1
+    a.level = lvl;
+    a.stampv = a.dev_it[1] + lvl;
+    a.launch = finalize ? 2 * lvl : 2 * lvl - 1;
+    a.qin = odd ? a.qb : a.qa;
+    a.sin = odd ? a.sb : a.sa;
+    a.qout = odd ? a.qa : a.qb;
+    a.sout = odd ? a.sa : a.sb;
+}
+__device__ __forceinline__ void plane_resolve(PlaneArgs& a) {
+    if (!a.dev_it) return;
+    const uint32_t it = a.dev_it[0];
+    const bool odd = ((it - 1) & 1) != 0;
+    a.it = it;
+    a.stampv = a.dev_it[1] + it;
+    a.qin = odd ? a.qb : a.qa;
+    a.sin = odd ? a.sb : a.sa;
+    a.qout = odd ? a.qa : a.qb;
+    a.sout = odd ? a.sa : a.sb;
+}
+
 // ---------------------------------------------------------------- multi-source BFS
 template <int K>
-__global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
+__global__ __launch_bounds__(WG) void k_ms_push(MsArgs args) {
+    MsArgs a = args;
+    ms_resolve(a, false);
     constexpr int VPR = WG / K, TV = TILE_EDGES / K, ROUNDS = TILE_EDGES / WG, EMAX = TV + 2;
     __shared__ uint32_t sS[EMAX];
     __shared__ uint64_t sRow[EMAX];
@@ -152,7 +179,9 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs a) {
 
 // Finalize: new = next & ~seen; seen |= new; visit = new; next = 0; record levels; queue the
 // vertices with new bits (and out-edges) as the next frontier.
-__global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
+__global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
+    MsArgs a = args;
+    ms_resolve(a, true);
     __shared__ uint64_t sScan[WG / WAVE + 1];
     __shared__ unsigned long long sBase;
     __shared__ unsigned long long sAct[MAXW];
@@ -223,7 +252,9 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs a) {
 // Best when many queries are in flight together; the two-pass kernel below is better when
 // few are.
 template <int K>
-__global__ __launch_bounds__(WG) void k_plane_push_wave(PlaneArgs a) {
+__global__ __launch_bounds__(WG) void k_plane_push_wave(PlaneArgs args) {
+    PlaneArgs a = args;
+    plane_resolve(a);
     constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE;
     __shared__ uint32_t sS[EMAX];
     __shared__ uint64_t sRow[EMAX];
@@ -392,7 +423,9 @@ constexpr uint32_t SPARSE_MAX = 8;
 constexpr uint32_t PLANE_DENSE_FROM = 16;
 
 template <int K>
-__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
+__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs args) {
+    PlaneArgs a = args;
+    plane_resolve(a);
     constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE, VPR = WG / K;
     constexpr int ROUNDS = TILE_EDGES / WG;
     __shared__ uint32_t sS[EMAX];
@@ -624,11 +657,21 @@ __global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs a) {
 
 // Vertices of the frontier just pushed forget their change masks; vertices improved this
 // iteration take theirs from nchg.  Runs over the previous frontier, then the new one.
-__global__ void k_plane_rotate(const uint32_t* __restrict__ q, const Ctl* ctl, int slot,
+// With dev_it (replayed graph) the queue, slot and activity slot come from the device iteration:
+// move = 0 takes the frontier the push read, move = 1 the one it wrote.
+__global__ void k_plane_rotate(const uint32_t* q, const Ctl* ctl, int slot,
                                uint64_t* __restrict__ from, uint64_t* __restrict__ to, uint32_t W,
-                               int move, uint64_t* __restrict__ act) {
+                               int move, uint64_t* act, const uint32_t* dev_it, const uint32_t* qa,
+                               const uint32_t* qb, uint64_t* act_base) {
     __shared__ unsigned long long sAct[MAXW];
     if (ctl->done) return;
+    if (dev_it) {
+        const uint32_t it = dev_it[0];
+        const bool odd = ((it - 1) & 1) != 0;
+        q = move ? (odd ? qa : qb) : (odd ? qb : qa);
+        slot = move ? (int)(it % 3) : (int)((it - 1) % 3);
+        act = (move && act_base) ? act_base + (it % 4) * MAXW : nullptr;
+    }
     if (threadIdx.x < MAXW) sAct[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t count = (uint32_t)(ctl->qpack[slot] & 0xffffffffull);
@@ -789,17 +832,123 @@ struct BatchWork {
     DevMem m1, m2, m3;      // seen / visit / next (BFS) or chg / nchg (planes)
     DevMem planes;          // n x B labels
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
+    DevMem devit;           // replayed chunks: {iteration, stamp base}
     uint32_t epoch = 1;
+    hipGraphExec_t exec = nullptr;  // graph of one chunk and the arguments it was built with
+    std::vector<uint8_t> key;
+    ~BatchWork() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
 };
 
+namespace {
+
+// Iterations per replayed chunk (a multiple of 2 and 3: queue ping-pong and control slots).
+constexpr uint32_t BATCH_CHUNK = 6;
+
+__global__ void k_tick_b(uint32_t* it) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *it += 1;
+}
+
+eeg_status add_node(hipGraph_t graph, hipGraphNode_t* prev, const void* fn, unsigned grid, unsigned block,
+                    void** args) {
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(fn);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(block);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    EEG_HIP(hipGraphAddKernelNode(&node, graph, *prev ? prev : nullptr, *prev ? 1 : 0, &kp));
+    *prev = node;
+    return EEG_OK;
+}
+
+// Graph of BATCH_CHUNK iterations (BFS: push, finalize; planes: push, two rotations; each
+// followed by a counter tick), built node by node and kept until its arguments change.
+eeg_status batch_graph(BatchWork& bw, bool bfs, uint32_t K, unsigned grid, const MsArgs& ma, const PlaneArgs& pa,
+                       uint32_t nsrc, hipGraphExec_t* out) {
+    const bool dense = nsrc > PLANE_DENSE_FROM;
+    std::vector<uint8_t> key(sizeof(MsArgs) + sizeof(PlaneArgs) + 16);
+    memcpy(key.data(), &ma, sizeof(MsArgs));
+    memcpy(key.data() + sizeof(MsArgs), &pa, sizeof(PlaneArgs));
+    const uint32_t tail[4] = {bfs ? 1u : 0u, K, grid, dense ? 1u : 0u};
+    memcpy(key.data() + sizeof(MsArgs) + sizeof(PlaneArgs), tail, 16);
+    if (bw.exec && bw.key == key) {
+        *out = bw.exec;
+        return EEG_OK;
+    }
+    if (bw.exec) {
+        (void)hipGraphExecDestroy(bw.exec);
+        bw.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    EEG_HIP(hipGraphCreate(&graph, 0));
+    MsArgs m = ma;
+    PlaneArgs p = pa;
+    uint32_t* itp = const_cast<uint32_t*>(bfs ? ma.dev_it : pa.dev_it);
+    void* margs[] = {&m};
+    void* pargs[] = {&p};
+    void* targs[] = {&itp};
+    // rotation arguments (queue, slot and activity slot resolved on the device)
+    const uint32_t* rq = nullptr;
+    const Ctl* rctl = pa.ctl;
+    int rslot = 0;
+    uint64_t* rnone = nullptr;
+    uint64_t* rfrom1 = pa.nchg;
+    uint64_t* rto = pa.chg;
+    uint32_t rW = pa.W;
+    int move0 = 0, move1 = 1;
+    uint64_t* ract = nullptr;
+    const uint32_t* rit = pa.dev_it;
+    const uint32_t* rqa = pa.qa;
+    const uint32_t* rqb = pa.qb;
+    uint64_t* ractb = pa.act;
+    void* r0args[] = {&rq, &rctl, &rslot, &rnone, &rto, &rW, &move0, &ract, &rit, &rqa, &rqb, &ractb};
+    void* r1args[] = {&rq, &rctl, &rslot, &rfrom1, &rto, &rW, &move1, &ract, &rit, &rqa, &rqb, &ractb};
+    const void* fpush =
+        bfs ? ((K == 4) ? (const void*)k_ms_push<4> : (K == 8) ? (const void*)k_ms_push<8>
+               : (K == 16) ? (const void*)k_ms_push<16> : (const void*)k_ms_push<32>)
+        : dense ? ((K == 4) ? (const void*)k_plane_push_wave<4> : (K == 8) ? (const void*)k_plane_push_wave<8>
+                   : (K == 16) ? (const void*)k_plane_push_wave<16> : (const void*)k_plane_push_wave<32>)
+                : ((K == 4) ? (const void*)k_plane_push<4> : (K == 8) ? (const void*)k_plane_push<8>
+                   : (K == 16) ? (const void*)k_plane_push<16> : (const void*)k_plane_push<32>);
+    hipGraphNode_t prev = nullptr;
+    eeg_status rc = EEG_OK;
+    for (uint32_t i = 0; i < BATCH_CHUNK && rc == EEG_OK; ++i) {
+        if (bfs) {
+            rc = add_node(graph, &prev, fpush, grid, WG, margs);
+            if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_ms_finalize, grid, WG, margs);
+        } else {
+            rc = add_node(graph, &prev, fpush, grid, WG, pargs);
+            if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_plane_rotate, 2048, WG, r0args);
+            if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_plane_rotate, 2048, WG, r1args);
+        }
+        if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_tick_b, 1, 64, targs);
+    }
+    if (rc != EEG_OK) {
+        (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    const hipError_t e = hipGraphInstantiate(&bw.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        bw.exec = nullptr;
+        return fail(EEG_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    bw.key = key;
+    *out = bw.exec;
+    return EEG_OK;
+}
+
+}  // namespace
+
+// Batched state of a graph handle, created on first use and freed with the handle.  Callers
+// hold the handle's lock.
 static BatchWork& batch_work(eeg_graph* g) {
-    static std::mutex mu;
-    static std::vector<std::pair<eeg_graph*, BatchWork*>> table;
-    std::lock_guard<std::mutex> lk(mu);
-    for (auto& e : table)
-        if (e.first == g) return *e.second;
-    table.emplace_back(g, new BatchWork());
-    return *table.back().second;
+    if (!g->batch_state) g->batch_state = std::make_shared<BatchWork>();
+    return *static_cast<BatchWork*>(g->batch_state.get());
 }
 
 eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
@@ -830,9 +979,10 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         EEG_TRY(bw.s[i].alloc((size_t)n * 4 + 4));
     }
     EEG_TRY(bw.cand.alloc((size_t)n * 4 + 4));
-    if (!bw.stamp.p) {
+    if (bw.stamp.bytes < (size_t)n * 4 + 4) {
         EEG_TRY(bw.stamp.alloc((size_t)n * 4 + 4));
         EEG_HIP(hipMemsetAsync(bw.stamp.p, 0, bw.stamp.bytes, s));
+        bw.epoch = 1;
     }
     if (bw.epoch > 0xf0000000u) {
         EEG_HIP(hipMemsetAsync(bw.stamp.p, 0, bw.stamp.bytes, s));
@@ -901,57 +1051,84 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     // BFS: one level = push + finalize; planes: one iteration = push + two mask rotations.
     // Chunks of 8 run ahead of the host's look at the control block.
     const uint32_t qcap = n + 1;
+    // argument blocks shared by every iteration; the per-iteration fields are set per launch,
+    // or derived on the device when a graph of BATCH_CHUNK iterations is replayed
+    MsArgs ma{};
+    PlaneArgs pa{};
+    if (bfs) {
+        ma.row = g->out.rowp();
+        ma.col = g->out.colp();
+        ma.vnc = t.vnc.as<uint32_t>();
+        ma.seen = bw.m1.as<uint64_t>();
+        ma.visit = bw.m2.as<uint64_t>();
+        ma.next = bw.m3.as<uint64_t>();
+        ma.stamp = bw.stamp.as<uint32_t>();
+        ma.cand = bw.cand.as<uint32_t>();
+        ma.dist = bw.planes.as<uint32_t>();
+        ma.ctl = ctl;
+        ma.W = W;
+        ma.B = B;
+        ma.n = n;
+        ma.qcap = qcap;
+    } else {
+        pa.row = g->out.rowp();
+        pa.col = g->out.colp();
+        pa.wgt = g->out.wp();
+        pa.vnc = t.vnc.as<uint32_t>();
+        pa.lab = bw.planes.as<uint32_t>();
+        pa.chg = bw.m1.as<uint64_t>();
+        pa.nchg = bw.m3.as<uint64_t>();
+        pa.stamp = bw.stamp.as<uint32_t>();
+        pa.ctl = ctl;
+        pa.W = W;
+        pa.B = B;
+        pa.n = n;
+        pa.qcap = qcap;
+        pa.widest = widest ? 1 : 0;
+    }
+    const bool replay = !o.timing;
+    hipGraphExec_t exec = nullptr;
+    if (replay) {
+        EEG_TRY(bw.devit.alloc(8));
+        uint32_t* dit = bw.devit.as<uint32_t>();
+        k_fill_u32b<<<1, 64, 0, s>>>(dit, 1, 1u);
+        k_fill_u32b<<<1, 64, 0, s>>>(dit + 1, 1, base);
+        EEG_HIP(hipGetLastError());
+        ma.dev_it = pa.dev_it = dit;
+        ma.qa = pa.qa = bw.q[0].as<uint32_t>();
+        ma.sa = pa.sa = bw.s[0].as<uint32_t>();
+        ma.qb = pa.qb = bw.q[1].as<uint32_t>();
+        ma.sb = pa.sb = bw.s[1].as<uint32_t>();
+        EEG_TRY(batch_graph(bw, bfs, K, grid, ma, pa, nsrc, &exec));
+    }
     uint32_t j = 1;
     while (true) {
-        const uint32_t chunk = 8;
+        if (replay) {
+            EEG_HIP(hipGraphLaunch(exec, s));
+            j += BATCH_CHUNK;
+            launches += BATCH_CHUNK;
+        }
+        const uint32_t chunk = replay ? 0 : 8;
         for (uint32_t c = 0; c < chunk; ++c, ++j, ++launches) {
             const int cur = (int)((j - 1) & 1);
             if (bfs) {
-                MsArgs ma{};
-                ma.row = g->out.rowp();
-                ma.col = g->out.colp();
-                ma.vnc = t.vnc.as<uint32_t>();
-                ma.seen = bw.m1.as<uint64_t>();
-                ma.visit = bw.m2.as<uint64_t>();
-                ma.next = bw.m3.as<uint64_t>();
-                ma.stamp = bw.stamp.as<uint32_t>();
-                ma.cand = bw.cand.as<uint32_t>();
-                ma.dist = bw.planes.as<uint32_t>();
-                ma.ctl = ctl;
-                ma.W = W;
-                ma.B = B;
-                ma.n = n;
-                ma.qcap = qcap;
-                ma.level = j;
-                ma.stampv = base + j;
-                ma.qin = bw.q[cur].as<uint32_t>();
-                ma.sin = bw.s[cur].as<uint32_t>();
-                ma.qout = bw.q[cur ^ 1].as<uint32_t>();
-                ma.sout = bw.s[cur ^ 1].as<uint32_t>();
-                EEG_TRY(launch_ms_level(ma, K, grid, s));
+                MsArgs a = ma;
+                a.level = j;
+                a.stampv = base + j;
+                a.qin = bw.q[cur].as<uint32_t>();
+                a.sin = bw.s[cur].as<uint32_t>();
+                a.qout = bw.q[cur ^ 1].as<uint32_t>();
+                a.sout = bw.s[cur ^ 1].as<uint32_t>();
+                EEG_TRY(launch_ms_level(a, K, grid, s));
             } else {
-                PlaneArgs pa{};
-                pa.row = g->out.rowp();
-                pa.col = g->out.colp();
-                pa.wgt = g->out.wp();
-                pa.vnc = t.vnc.as<uint32_t>();
-                pa.lab = bw.planes.as<uint32_t>();
-                pa.chg = bw.m1.as<uint64_t>();
-                pa.nchg = bw.m3.as<uint64_t>();
-                pa.stamp = bw.stamp.as<uint32_t>();
-                pa.ctl = ctl;
-                pa.W = W;
-                pa.B = B;
-                pa.n = n;
-                pa.qcap = qcap;
-                pa.it = j;
-                pa.stampv = base + j;
-                pa.widest = widest ? 1 : 0;
-                pa.qin = bw.q[cur].as<uint32_t>();
-                pa.sin = bw.s[cur].as<uint32_t>();
-                pa.qout = bw.q[cur ^ 1].as<uint32_t>();
-                pa.sout = bw.s[cur ^ 1].as<uint32_t>();
-                EEG_TRY(launch_plane_iteration(pa, K, grid, s, nsrc));
+                PlaneArgs a = pa;
+                a.it = j;
+                a.stampv = base + j;
+                a.qin = bw.q[cur].as<uint32_t>();
+                a.sin = bw.s[cur].as<uint32_t>();
+                a.qout = bw.q[cur ^ 1].as<uint32_t>();
+                a.sout = bw.s[cur ^ 1].as<uint32_t>();
+                EEG_TRY(launch_plane_iteration(a, K, grid, s, nsrc));
             }
         }
         EEG_HIP(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -1013,9 +1190,11 @@ eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStr
         fn = (K == 4) ? k_plane_push_wave<4> : (K == 8) ? k_plane_push_wave<8> : (K == 16) ? k_plane_push_wave<16> : k_plane_push_wave<32>;
     EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
     // forget the pushed frontier's masks, then move this iteration's improvements in
-    k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr);
+    k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr);
     k_plane_rotate<<<2048, WG, 0, s>>>(a.qout, a.ctl, (int)(a.it % 3), a.nchg, a.chg, a.W, 1,
-                                        a.act ? a.act + (a.it % 4) * MAXW : nullptr);
+                                        a.act ? a.act + (a.it % 4) * MAXW : nullptr, nullptr, nullptr, nullptr,
+                                        nullptr);
     EEG_HIP(hipGetLastError());
     return EEG_OK;
 }

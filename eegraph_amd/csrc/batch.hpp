@@ -32,6 +32,10 @@ struct MsArgs {
     uint32_t launch;       // launch index (slot rotation)
     uint32_t level;        // level being discovered by this push (1-based)
     uint32_t stampv;
+    // replayed graph: level = dev_it[0], stamp base = dev_it[1]; launch, stampv and the queue
+    // ping-pong over (qa, sa) / (qb, sb) are derived from them
+    const uint32_t* dev_it;
+    uint32_t *qa, *sa, *qb, *sb;
 };
 
 // Batched SSSP / SSWP over vertex-major label planes.
@@ -56,6 +60,8 @@ struct PlaneArgs {
     uint32_t it;
     uint32_t stampv;
     int widest;           // 0: SSSP (min, +w), 1: SSWP (max, min(.,w))
+    const uint32_t* dev_it;  // replayed graph, as in MsArgs
+    uint32_t *qa, *sa, *qb, *sb;
 };
 
 // One BFS level: push then finalize (launch indices 2*level-1 and 2*level).

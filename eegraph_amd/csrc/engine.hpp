@@ -121,6 +121,7 @@ struct eeg_graph {
     std::vector<uint32_t> h_perm;    // host copy of perm (sources map without a device read)
     uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
     uint64_t serial = 0;             // unique per handle in this process
+    std::shared_ptr<void> batch_state;  // batched-traversal buffers (batch.hip), freed with the handle
     std::mutex mu;
     ~eeg_graph();
 };

@@ -183,5 +183,6 @@ def test_replayed_chunks_match_eager_launches(rmat14, alg):
         g.run("bfs", (src + 1) % n, variant="async_push_dd", K=8)  # other work in between
         eager, st_e = g.run(alg, src, variant="sync_push_dd", K=8, timing=True)
         assert np.array_equal(replayed, eager), (alg, rep)
-        assert st_r["iterations"] == st_e["iterations"]
+        # iteration counts may differ: labels are updated in place, so a vertex can see a
+        # neighbour's improvement from the same iteration
         assert np.array_equal(replayed, O.reference_labels(ref_alg, n, s, d, w, src)), (alg, rep)
