@@ -8,6 +8,8 @@
 #include "oracle.h"
 
 #include <stdlib.h>
+#include <math.h>
+#include <omp.h>
 #include <string.h>
 #include <time.h>
 
@@ -346,6 +348,37 @@ uint32_t orc_pagerank(uint32_t n, const uint64_t* row, const uint32_t* col, doub
     }
     if (residual) *residual = res;
     free(nx);
+    return it;
+}
+
+uint32_t orc_pagerank_pull(uint32_t n, const uint64_t* in_row, const uint32_t* in_col,
+                           const uint32_t* outdeg, double d, double eps, uint32_t max_iter,
+                           int threads, double* x, double* residual) {
+    double* nx = (double*)malloc(sizeof(double) * ((size_t)n + 1));
+    double* c = (double*)malloc(sizeof(double) * ((size_t)n + 1));
+    if (threads > 0) omp_set_num_threads(threads);
+    const double base = (1.0 - d) / n;
+    for (uint32_t v = 0; v < n; ++v) x[v] = 1.0 / n;
+    uint32_t it = 0;
+    double res = 0;
+    while (it < max_iter) {
+#pragma omp parallel for schedule(static)
+        for (int64_t v = 0; v < (int64_t)n; ++v) c[v] = outdeg[v] ? d * x[v] / (double)outdeg[v] : 0.0;
+        res = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : res)
+        for (int64_t v = 0; v < (int64_t)n; ++v) {
+            double s = base;
+            for (uint64_t e = in_row[v]; e < in_row[v + 1]; ++e) s += c[in_col[e]];
+            nx[v] = s;
+            res += fabs(s - x[v]);
+        }
+        memcpy(x, nx, sizeof(double) * n);
+        ++it;
+        if (res < eps) break;
+    }
+    if (residual) *residual = res;
+    free(nx);
+    free(c);
     return it;
 }
 

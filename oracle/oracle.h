@@ -67,6 +67,15 @@ uint32_t orc_pagerank(uint32_t n, const uint64_t* row, const uint32_t* col, doub
                       uint32_t max_iter, int personalised, uint32_t source, double* x,
                       double* residual);
 
+/* The same FP64 Jacobi iteration (not personalised) in pull form over the in-edge CSR
+ * (in_row/in_col: sources of each vertex's in-edges; outdeg: out-degree of every vertex),
+ * parallel over destinations with OpenMP (`threads` <= 0: the OpenMP default).  Each vertex's
+ * sum is taken in in-edge order by one thread, so the ranks do not depend on the thread
+ * count.  For large graphs, where orc_pagerank would take minutes. */
+uint32_t orc_pagerank_pull(uint32_t n, const uint64_t* in_row, const uint32_t* in_col,
+                           const uint32_t* outdeg, double d, double eps, uint32_t max_iter,
+                           int threads, double* x, double* residual);
+
 /* Serial delta-push PageRank following src/pr.cu:171-225 (value += delta, push d*delta/deg)
  * with D6 fixed (each vertex consumes its delta once) and in the normalised scale
  * (x0 = 0, residual0 = (1-d)/N).  Stops when no vertex holds residual > thr.  Float state

@@ -44,6 +44,8 @@ _sig("orc_wcc", None, C.c_uint32, C.c_uint64, _vp, _vp, _vp)
 _sig("orc_brute", C.c_int, C.c_int, C.c_uint32, C.c_uint64, _vp, _vp, _vp, C.c_uint32, _vp)
 _sig("orc_pagerank", C.c_uint32, C.c_uint32, _vp, _vp, C.c_double, C.c_double, C.c_uint32, C.c_int,
      C.c_uint32, _vp, C.POINTER(C.c_double))
+_sig("orc_pagerank_pull", C.c_uint32, C.c_uint32, _vp, _vp, _vp, C.c_double, C.c_double, C.c_uint32,
+     C.c_int, _vp, C.POINTER(C.c_double))
 _sig("orc_pr_delta_push", C.c_uint32, C.c_uint32, _vp, _vp, C.c_float, C.c_float, _vp, C.c_double,
      C.POINTER(C.c_int))
 
@@ -132,6 +134,15 @@ def pagerank(n, row, col, d=0.85, eps=1e-12, max_iter=10000, personalised=False,
     res = C.c_double(0)
     it = _lib.orc_pagerank(n, _p(row), _p(col), d, eps, max_iter, 1 if personalised else 0,
                            source, _p(x), C.byref(res))
+    return x, it, res.value
+
+
+def pagerank_pull(n, in_row, in_col, outdeg, d=0.85, eps=1e-12, max_iter=10000, threads=0):
+    """FP64 Jacobi PageRank over the in-edge CSR, OpenMP over destinations."""
+    x = np.empty(n, np.float64)
+    res = C.c_double(0)
+    it = _lib.orc_pagerank_pull(n, _p(in_row), _p(in_col), _p(outdeg), d, eps, max_iter, threads,
+                                _p(x), C.byref(res))
     return x, it, res.value
 
 

@@ -173,3 +173,19 @@ def test_rmat14_fixture_and_oracle_labels():
     for name, code in (("bfs", O.BFS), ("sssp", O.SSSP), ("sswp", O.SSWP), ("cc", O.CC)):
         lab = O.reference_labels(code, n, s, d, w, src)
         assert hashlib.sha256(lab.tobytes()).hexdigest() == g["labels_sha256"][name]
+
+
+def test_pagerank_pull_equals_push_form():
+    """The parallel pull form of the FP64 oracle equals the serial push form (itself pinned by
+    closed forms) on an RMAT graph with dangling vertices, for any thread count."""
+    import eegraph_amd as eg
+    s, d, w = eg.gen_rmat(12, 8, seed=3)
+    n = 1 << 12
+    row, col, _ = O.csr(n, s, d)
+    irow, icol, _ = O.csr(n, d, s)
+    outdeg = np.diff(row).astype(np.uint32)
+    xp, itp, _ = O.pagerank(n, row, col, 0.85, 1e-12)
+    for threads in (1, 3, 8):
+        xq, itq, _ = O.pagerank_pull(n, irow, icol, outdeg, 0.85, 1e-12, threads=threads)
+        assert itq == itp
+        assert np.abs(xq - xp).max() < 1e-15 and np.abs(xq - xp).sum() < 1e-12
