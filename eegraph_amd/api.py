@@ -230,7 +230,8 @@ class Graph:
         o = opts if opts is not None else make_opts(**kw)
         srcs = _u32(np.asarray(sources))
         n = self.info()["n"]
-        out = np.empty((len(srcs), n), np.uint32) if want_labels else None
+        out = (np.empty((len(srcs), n), np.float32 if a == L.PPR else np.uint32)
+               if want_labels else None)
         st = L.Stats()
         L.check(L.lib.eeg_run_batch(self._h, a, _ptr(srcs), len(srcs), C.byref(o), _ptr(out),
                                     C.byref(st)), f"eeg_run_batch({algo})")

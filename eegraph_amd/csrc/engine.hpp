@@ -174,4 +174,5 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
                          Work* wk = nullptr, hipStream_t sk = nullptr);
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st, Work* wk = nullptr, hipStream_t sk = nullptr);
+
 }  // namespace eeg

@@ -221,8 +221,9 @@ eeg_status eeg_run_device(eeg_graph* g, uint32_t algo, uint32_t source, const ee
 
 /* ---------------- batched queries ----------------
  * nsrc sources in one traversal: BFS runs multi-source BFS with 64-bit visit masks
- * (ceil(nsrc/64) words per vertex); SSSP/SSWP run vertex-major label planes.  labels_out is
- * nsrc x n, row q is the answer for sources[q], identical to eeg_run(sources[q]). */
+ * (ceil(nsrc/64) words per vertex, passes of 256); SSSP/SSWP run vertex-major label planes
+ * (passes of 256); PPR runs one pull iteration per source.  labels_out is nsrc x n (u32, or
+ * f32 for PPR), row q the answer for sources[q], identical to eeg_run(sources[q]). */
 eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc,
                          const eeg_opts* opts, void* labels_out, eeg_stats* stats);
 /* Several graphs in one launch sequence: graphs[i] with sources[i]; labels_out[i] gets n_i

@@ -148,3 +148,36 @@ def test_pagerank_rmat24_within_1e6_of_fp64():
         l1 = np.abs(x.astype(np.float64) - xf).sum()
         assert l1 < L1_TOL, (var, l1)
         assert np.abs(x.astype(np.float64) - xf).max() < 1e-7, var
+
+
+@pytest.mark.parametrize("nsrc", [1, 5, 8, 13])
+@pytest.mark.parametrize("K", [8, 32])
+def test_batched_ppr_vs_single_and_fp64(rmat14, nsrc, K):
+    """Batched PPR rows are within the PageRank tolerance of the single-query run and of the
+    FP64 oracle."""
+    n, s, d, g, row, col = rmat14
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.integers(0, n, nsrc).astype(np.uint32)
+    srcs[0] = 0
+    out, st = g.run_batch("ppr", srcs, K=K, epsilon=1e-7)
+    assert out.shape == (nsrc, n) and out.dtype == np.float32
+    assert st["pr_residual"] < 1e-7
+    for q, src in enumerate(srcs):
+        single, _ = g.run("ppr", int(src), K=K, epsilon=1e-7)
+        assert np.abs(out[q].astype(np.float64) - single).sum() < L1_TOL, (q, src)
+        xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(src))
+        assert np.abs(out[q].astype(np.float64) - xf).sum() < L1_TOL, (q, src)
+
+
+def test_batched_ppr_isolated_and_repeated_sources():
+    n = 60
+    s = np.array([1, 2, 3, 3], np.uint32)
+    d = np.array([2, 3, 1, 4], np.uint32)
+    g = eg.Graph.from_edges(s, d, None, n=n)
+    srcs = np.array([5, 1, 1, 4, 59], np.uint32)
+    out, _ = g.run_batch("ppr", srcs, K=4)
+    for q, src in enumerate(srcs):
+        single, _ = g.run("ppr", int(src), K=4)
+        assert np.abs(out[q].astype(np.float64) - single).sum() < 1e-7, q
+    assert abs(out[0][5] - 0.15) < 1e-7  # restart mass stays on a vertex without out-edges
+    g.close()
