@@ -124,30 +124,69 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs args) {
             sOwn[j] = (uint16_t)lo;
         }
         __syncthreads();
+        // edge loads of every round first; then, word by word, every visit and seen load of
+        // all rounds before any atomic, so each lane keeps ROUNDS requests in flight
+        uint32_t ru[ROUNDS], rv[ROUNDS];
+        bool rok[ROUNDS], got[ROUNDS];
+#pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
             const int slot = r * VPR + grp;
             const uint32_t t = t0 + slot;
-            bool got = false;
-            uint32_t u = 0;
+            rok[r] = false;
+            got[r] = false;
+            ru[r] = 0;
+            rv[r] = 0;
             if (t < t1) {
                 const int k = sOwn[slot];
                 const uint32_t eoff = (t - sS[k]) * K + lj;
                 if (eoff < sDeg[k]) {
-                    u = a.col[sRow[k] + eoff];
-                    const uint32_t v = sV[k];
-                    for (uint32_t w = 0; w < a.W; ++w) {
-                        const uint64_t m = a.visit[(uint64_t)v * a.W + w];
-                        if (!m) continue;
-                        const uint64_t add = m & ~a.seen[(uint64_t)u * a.W + w];
-                        if (add) {
-                            atomicOr((unsigned long long*)&a.next[(uint64_t)u * a.W + w],
-                                     (unsigned long long)add);
-                            got = true;
-                        }
+                    rok[r] = true;
+                    rv[r] = sV[k];
+                    ru[r] = __builtin_nontemporal_load(&a.col[sRow[k] + eoff]);
+                }
+            }
+        }
+        if ((a.W & 1) == 0) {
+            // even W: the masks of a vertex are 16-byte aligned, two words per load
+            for (uint32_t w = 0; w < a.W; w += 2) {
+                ulonglong2 mv[ROUNDS], ms[ROUNDS];
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r)
+                    mv[r] = rok[r] ? *(const ulonglong2*)&a.visit[(uint64_t)rv[r] * a.W + w] : ulonglong2{0, 0};
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r)
+                    ms[r] = (mv[r].x | mv[r].y) ? *(const ulonglong2*)&a.seen[(uint64_t)ru[r] * a.W + w]
+                                                : ulonglong2{~0ull, ~0ull};
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const uint64_t add0 = mv[r].x & ~ms[r].x, add1 = mv[r].y & ~ms[r].y;
+                    unsigned long long* nx = (unsigned long long*)&a.next[(uint64_t)ru[r] * a.W + w];
+                    if (add0) atomicOr(nx, (unsigned long long)add0);
+                    if (add1) atomicOr(nx + 1, (unsigned long long)add1);
+                    got[r] = got[r] || (add0 | add1) != 0;
+                }
+            }
+        } else {
+            for (uint32_t w = 0; w < a.W; ++w) {
+                uint64_t mv[ROUNDS], ms[ROUNDS];
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) mv[r] = rok[r] ? a.visit[(uint64_t)rv[r] * a.W + w] : 0ull;
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) ms[r] = mv[r] ? a.seen[(uint64_t)ru[r] * a.W + w] : ~0ull;
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const uint64_t add = mv[r] & ~ms[r];
+                    if (add) {
+                        atomicOr((unsigned long long*)&a.next[(uint64_t)ru[r] * a.W + w], (unsigned long long)add);
+                        got[r] = true;
                     }
                 }
             }
-            bool candq = got && atomicExch(&a.stamp[u], a.stampv) != a.stampv;
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const uint32_t u = ru[r];
+            const bool candq = got[r] && atomicExch(&a.stamp[u], a.stampv) != a.stampv;
             const uint64_t mask = __ballot(candq);
             if (mask) {
                 const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -178,13 +217,17 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs args) {
 }
 
 // Finalize: new = next & ~seen; seen |= new; visit = new; next = 0; record levels; queue the
-// vertices with new bits (and out-edges) as the next frontier.
+// vertices with new bits (and out-edges) as the next frontier.  A thread per candidate updates
+// the masks; the levels are then written by whole wavefronts, lane = query, one contiguous row
+// store per (candidate, word) with new bits, instead of one scattered store per new bit.
 __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
     MsArgs a = args;
     ms_resolve(a, true);
     __shared__ uint64_t sScan[WG / WAVE + 1];
     __shared__ unsigned long long sBase;
     __shared__ unsigned long long sAct[MAXW];
+    __shared__ uint64_t sNew[MAXW][WG];
+    __shared__ uint32_t sU[WG];
     Ctl* ctl = a.ctl;
     if (ctl->done) return;
     if (threadIdx.x < MAXW) sAct[threadIdx.x] = 0;
@@ -196,30 +239,25 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
         return;
     }
     if (lead) ctl->qpack[(a.launch + 1) % 3] = 0;
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
     for (uint64_t base = (uint64_t)blockIdx.x * WG; base < count; base += (uint64_t)gridDim.x * WG) {
         const uint64_t i = base + threadIdx.x;
         uint64_t mine = 0;
         uint32_t u = 0, nv = 0;
+        bool any = false;
         if (i < count) {
             u = a.cand[i];
-            bool any = false;
             const bool pushes = a.vnc[u] != 0;
             for (uint32_t w = 0; w < a.W; ++w) {
                 const uint64_t idx = (uint64_t)u * a.W + w;
                 const uint64_t nw = a.next[idx] & ~a.seen[idx];
                 a.next[idx] = 0;
                 a.visit[idx] = nw;
+                sNew[w][threadIdx.x] = nw;
                 if (nw) {
                     a.seen[idx] |= nw;
                     any = true;
                     if (a.act && pushes) atomicOr(&sAct[w], (unsigned long long)nw);
-                    uint64_t bits = nw;
-                    while (bits) {
-                        const int b = __ffsll((unsigned long long)bits) - 1;
-                        bits &= bits - 1;
-                        const uint32_t q = w * 64 + (uint32_t)b;
-                        if (q < a.B) a.dist[(uint64_t)u * a.B + q] = a.level;
-                    }
                 }
             }
             if (any) {
@@ -227,6 +265,7 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
                 if (nv) mine = 1ull | ((uint64_t)nv << 32);
             }
         }
+        sU[threadIdx.x] = any ? u : 0xffffffffu;
         uint64_t tot;
         const uint64_t pre = block_excl_scan<uint64_t>(mine, sScan, &tot);
         if (threadIdx.x == 0) sBase = tot ? atomicAdd(&ctl->qpack[a.launch % 3], (unsigned long long)tot) : 0ull;
@@ -238,6 +277,17 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
                 a.sout[pos] = (uint32_t)(sBase >> 32) + (uint32_t)(pre >> 32);
             } else {
                 atomicOr(&ctl->err, 8u);
+            }
+        }
+        // levels of this wavefront's candidates
+        for (int c = 0; c < WAVE; ++c) {
+            const int slot = wv * WAVE + c;
+            const uint32_t cu = sU[slot];
+            if (cu == 0xffffffffu) continue;
+            for (uint32_t w = 0; w < a.W; ++w) {
+                const uint64_t nw = sNew[w][slot];
+                const uint32_t q = w * 64 + (uint32_t)lane;
+                if (((nw >> lane) & 1ull) && q < a.B) a.dist[(uint64_t)cu * a.B + q] = a.level;
             }
         }
         __syncthreads();
@@ -702,13 +752,33 @@ __global__ void k_planes_out(const uint32_t* __restrict__ planes, const uint32_t
     }
 }
 
+// Fills with 16-byte stores where the base allows (the planes are many GB).
 __global__ void k_fill_u32b(uint32_t* __restrict__ a, uint64_t n, uint32_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        a[i] = v;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t head = 0;
+    if (((uintptr_t)a & 15) == 0) {
+        const uint64_t n4 = n / 4;
+        uint4* a4 = reinterpret_cast<uint4*>(a);
+        for (uint64_t i = tid; i < n4; i += nt) a4[i] = make_uint4(v, v, v, v);
+        head = n4 * 4;
+    }
+    for (uint64_t i = head + tid; i < n; i += nt) a[i] = v;
 }
 __global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        a[i] = 0;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t head = 0;
+    if (((uintptr_t)a & 15) == 0) {
+        const uint64_t n2 = n / 2;
+        ulonglong2* a2 = reinterpret_cast<ulonglong2*>(a);
+        for (uint64_t i = tid; i < n2; i += nt) a2[i] = ulonglong2{0, 0};
+        head = n2 * 2;
+    }
+    for (uint64_t i = head + tid; i < n; i += nt) a[i] = 0;
+}
+// dst[i] = src[idx[i]]
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t m,
+                             uint32_t* __restrict__ dst) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) dst[i] = src[idx[i]];
 }
 
 // Seeds: entries (vertex, query) applied on the device; the first frontier is the distinct
@@ -1002,9 +1072,15 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     std::vector<uint32_t> fv(sv);
     std::sort(fv.begin(), fv.end());
     fv.erase(std::unique(fv.begin(), fv.end()), fv.end());
+    uint32_t* dseed = bw.seeds.as<uint32_t>();
+    // virtual-node counts of the distinct sources, gathered into the seed buffer's frontier
+    // half (overwritten below) and read back in one copy
     std::vector<uint32_t> hvnc(fv.size());
-    for (size_t i = 0; i < fv.size(); ++i)
-        EEG_HIP(hipMemcpyAsync(&hvnc[i], t.vnc.as<uint32_t>() + fv[i], 4, hipMemcpyDeviceToHost, s));
+    EEG_HIP(hipMemcpyAsync(dseed + 2 * nsrc, fv.data(), fv.size() * 4, hipMemcpyHostToDevice, s));
+    k_gather_u32<<<grid_n(fv.size()), WG, 0, s>>>(t.vnc.as<uint32_t>(), dseed + 2 * nsrc, (uint32_t)fv.size(),
+                                                 dseed + 3 * nsrc);
+    EEG_HIP(hipGetLastError());
+    EEG_HIP(hipMemcpyAsync(hvnc.data(), dseed + 3 * nsrc, fv.size() * 4, hipMemcpyDeviceToHost, s));
     EEG_HIP(hipStreamSynchronize(s));
     std::vector<uint32_t> fvv, fss;
     uint32_t acc = 0;
@@ -1014,7 +1090,6 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
             fss.push_back(acc);
             acc += hvnc[i];
         }
-    uint32_t* dseed = bw.seeds.as<uint32_t>();
     EEG_HIP(hipMemcpyAsync(dseed, sv.data(), nsrc * 4, hipMemcpyHostToDevice, s));
     EEG_HIP(hipMemcpyAsync(dseed + nsrc, sq.data(), nsrc * 4, hipMemcpyHostToDevice, s));
     if (!fvv.empty()) {

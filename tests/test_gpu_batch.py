@@ -21,7 +21,7 @@ def rmat13():
 
 
 @pytest.mark.parametrize("alg", list(ALG))
-@pytest.mark.parametrize("nsrc", [1, 37, 64, 200, 256, 300])
+@pytest.mark.parametrize("nsrc", [1, 37, 64, 100, 150, 200, 256, 300])
 def test_batch_equals_single_and_oracle(rmat13, alg, nsrc):
     n, s, d, w, g = rmat13
     rng = np.random.default_rng(nsrc)
