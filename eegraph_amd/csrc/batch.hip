@@ -146,21 +146,35 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs args) {
                 }
             }
         }
+        // With interleaved records (R != W) the destination's `next` words and stamp share the
+        // cache line of its `seen` words, so a snapshot of them comes with the seen load, and the
+        // atomics are issued only for bits and stamps it does not show yet.  Bits of `next` are
+        // only added during a push and a stamp only moves to this level's value, so a stale
+        // snapshot costs an extra atomic and never a lost bit.  (The atomics are what this
+        // kernel waits on: they are performed beyond the L2.)
+        const bool snap = a.R != a.W;
+        uint32_t sp[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) sp[r] = 0;
         if ((a.W & 1) == 0) {
             // even W: the masks of a vertex are 16-byte aligned, two words per load
             for (uint32_t w = 0; w < a.W; w += 2) {
-                ulonglong2 mv[ROUNDS], ms[ROUNDS];
+                ulonglong2 mv[ROUNDS], ms[ROUNDS], mn[ROUNDS];
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r)
-                    mv[r] = rok[r] ? *(const ulonglong2*)&a.visit[(uint64_t)rv[r] * a.W + w] : ulonglong2{0, 0};
-#pragma unroll
-                for (int r = 0; r < ROUNDS; ++r)
-                    ms[r] = (mv[r].x | mv[r].y) ? *(const ulonglong2*)&a.seen[(uint64_t)ru[r] * a.W + w]
-                                                : ulonglong2{~0ull, ~0ull};
+                    mv[r] = rok[r] ? *(const ulonglong2*)&a.visit[(uint64_t)rv[r] * a.R + w] : ulonglong2{0, 0};
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
-                    const uint64_t add0 = mv[r].x & ~ms[r].x, add1 = mv[r].y & ~ms[r].y;
-                    unsigned long long* nx = (unsigned long long*)&a.next[(uint64_t)ru[r] * a.W + w];
+                    const bool any = (mv[r].x | mv[r].y) != 0;
+                    const uint64_t i = (uint64_t)ru[r] * a.R + w;
+                    ms[r] = any ? *(const ulonglong2*)&a.seen[i] : ulonglong2{~0ull, ~0ull};
+                    mn[r] = (any && snap) ? *(const ulonglong2*)&a.next[i] : ulonglong2{0, 0};
+                    if (w == 0 && any && snap) sp[r] = a.stamp[(uint64_t)ru[r] * a.sst];
+                }
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const uint64_t add0 = mv[r].x & ~ms[r].x & ~mn[r].x, add1 = mv[r].y & ~ms[r].y & ~mn[r].y;
+                    unsigned long long* nx = (unsigned long long*)&a.next[(uint64_t)ru[r] * a.R + w];
                     if (add0) atomicOr(nx, (unsigned long long)add0);
                     if (add1) atomicOr(nx + 1, (unsigned long long)add1);
                     got[r] = got[r] || (add0 | add1) != 0;
@@ -168,25 +182,32 @@ __global__ __launch_bounds__(WG) void k_ms_push(MsArgs args) {
             }
         } else {
             for (uint32_t w = 0; w < a.W; ++w) {
-                uint64_t mv[ROUNDS], ms[ROUNDS];
+                uint64_t mv[ROUNDS], ms[ROUNDS], mn[ROUNDS];
 #pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) mv[r] = rok[r] ? a.visit[(uint64_t)rv[r] * a.W + w] : 0ull;
-#pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) ms[r] = mv[r] ? a.seen[(uint64_t)ru[r] * a.W + w] : ~0ull;
+                for (int r = 0; r < ROUNDS; ++r) mv[r] = rok[r] ? a.visit[(uint64_t)rv[r] * a.R + w] : 0ull;
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
-                    const uint64_t add = mv[r] & ~ms[r];
+                    const uint64_t i = (uint64_t)ru[r] * a.R + w;
+                    ms[r] = mv[r] ? a.seen[i] : ~0ull;
+                    mn[r] = (mv[r] && snap) ? a.next[i] : 0ull;
+                    if (w == 0 && mv[r] && snap) sp[r] = a.stamp[(uint64_t)ru[r] * a.sst];
+                }
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const uint64_t add = mv[r] & ~ms[r] & ~mn[r];
                     if (add) {
-                        atomicOr((unsigned long long*)&a.next[(uint64_t)ru[r] * a.W + w], (unsigned long long)add);
+                        atomicOr((unsigned long long*)&a.next[(uint64_t)ru[r] * a.R + w], (unsigned long long)add);
                         got[r] = true;
                     }
                 }
             }
         }
+        // a destination that gained bits is listed once per level (its stamp)
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
             const uint32_t u = ru[r];
-            const bool candq = got[r] && atomicExch(&a.stamp[u], a.stampv) != a.stampv;
+            const bool candq = got[r] && sp[r] != a.stampv &&
+                               atomicExch(&a.stamp[(uint64_t)u * a.sst], a.stampv) != a.stampv;
             const uint64_t mask = __ballot(candq);
             if (mask) {
                 const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -249,7 +270,7 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
             u = a.cand[i];
             const bool pushes = a.vnc[u] != 0;
             for (uint32_t w = 0; w < a.W; ++w) {
-                const uint64_t idx = (uint64_t)u * a.W + w;
+                const uint64_t idx = (uint64_t)u * a.R + w;
                 const uint64_t nw = a.next[idx] & ~a.seen[idx];
                 a.next[idx] = 0;
                 a.visit[idx] = nw;
@@ -785,9 +806,9 @@ __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* _
 // source vertices with their virtual-node prefix, computed on the host.
 __global__ void k_seed(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ sq, uint32_t ns,
                        uint64_t* __restrict__ m1, uint64_t* __restrict__ m2, uint32_t* __restrict__ lab,
-                       uint32_t W, uint32_t B, uint32_t val) {
+                       uint32_t R, uint32_t B, uint32_t val) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
-        const uint64_t idx = (uint64_t)sv[i] * W + sq[i] / 64;
+        const uint64_t idx = (uint64_t)sv[i] * R + sq[i] / 64;
         const uint64_t bit = 1ull << (sq[i] % 64);
         atomicOr((unsigned long long*)&m1[idx], (unsigned long long)bit);
         if (m2) atomicOr((unsigned long long*)&m2[idx], (unsigned long long)bit);
@@ -899,7 +920,7 @@ int cu_count_b(int device) {
 
 // Batched state of a graph handle (allocated on first use, grown as needed).
 struct BatchWork {
-    DevMem m1, m2, m3;      // seen / visit / next (BFS) or chg / nchg (planes)
+    DevMem m1, m3;          // BFS: per-vertex mask records (m1); planes: chg (m1), nchg (m3)
     DevMem planes;          // n x B labels
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
     DevMem devit;           // replayed chunks: {iteration, stamp base}
@@ -1040,9 +1061,14 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     BatchWork& bw = batch_work(g);
     const uint32_t W = (nsrc + 63) / 64, B = W * 64;
     const uint64_t nw = (uint64_t)n * W, nb = (uint64_t)n * B;
-    EEG_TRY(bw.m1.alloc(nw * 8 + 8));
-    EEG_TRY(bw.m2.alloc(nw * 8 + 8));
-    EEG_TRY(bw.m3.alloc(nw * 8 + 8));
+    const bool bfs = (algo == EEG_BFS);
+    const bool widest = (algo == EEG_SSWP);
+    // BFS: one record of 4W words per vertex in m1 (seen, next, visit, stamp).  Planes: change
+    // masks chg (m1) and nchg (m3), W words per vertex.
+    const uint32_t R = bfs ? 4 * W : W;
+    const uint64_t nrec = (uint64_t)n * R;
+    EEG_TRY(bw.m1.alloc(nrec * 8 + 8));
+    if (!bfs) EEG_TRY(bw.m3.alloc(nw * 8 + 8));
     EEG_TRY(bw.planes.alloc(nb * 4 + 4));
     for (int i = 0; i < 2; ++i) {
         EEG_TRY(bw.q[i].alloc((size_t)n * 4 + 4));
@@ -1096,17 +1122,14 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         EEG_HIP(hipMemcpyAsync(dseed + 2 * nsrc, fvv.data(), fvv.size() * 4, hipMemcpyHostToDevice, s));
         EEG_HIP(hipMemcpyAsync(dseed + 3 * nsrc, fss.data(), fss.size() * 4, hipMemcpyHostToDevice, s));
     }
-    const bool bfs = (algo == EEG_BFS);
-    const bool widest = (algo == EEG_SSWP);
-    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m1.as<uint64_t>(), nw);
-    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m2.as<uint64_t>(), nw);
-    k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m3.as<uint64_t>(), nw);
+    uint64_t* rec = bw.m1.as<uint64_t>();
+    k_fill_u64<<<grid_n(nrec), WG, 0, s>>>(rec, nrec);
+    if (!bfs) k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m3.as<uint64_t>(), nw);
     k_fill_u32b<<<grid_n(nb), WG, 0, s>>>(bw.planes.as<uint32_t>(), nb, widest ? 0u : INF);
     EEG_HIP(hipGetLastError());
     // BFS: seen and visit get the source bits, level 0.  Planes: chg gets them.
-    k_seed<<<grid_n(nsrc), WG, 0, s>>>(dseed, dseed + nsrc, nsrc, bw.m1.as<uint64_t>(),
-                                       bfs ? bw.m2.as<uint64_t>() : nullptr, bw.planes.as<uint32_t>(), W, B,
-                                       widest ? INF : 0u);
+    k_seed<<<grid_n(nsrc), WG, 0, s>>>(dseed, dseed + nsrc, nsrc, rec, bfs ? rec + 2 * W : nullptr,
+                                       bw.planes.as<uint32_t>(), R, B, widest ? INF : 0u);
     k_set_frontier<<<grid_n(fvv.size() ? fvv.size() : 1), WG, 0, s>>>(
         dseed + 2 * nsrc, dseed + 3 * nsrc, (uint32_t)fvv.size(),
         ((unsigned long long)acc << 32) | (unsigned long long)fvv.size(), bw.q[0].as<uint32_t>(),
@@ -1134,10 +1157,12 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         ma.row = g->out.rowp();
         ma.col = g->out.colp();
         ma.vnc = t.vnc.as<uint32_t>();
-        ma.seen = bw.m1.as<uint64_t>();
-        ma.visit = bw.m2.as<uint64_t>();
-        ma.next = bw.m3.as<uint64_t>();
-        ma.stamp = bw.stamp.as<uint32_t>();
+        ma.seen = rec;
+        ma.next = rec + W;
+        ma.visit = rec + 2 * W;
+        ma.stamp = reinterpret_cast<uint32_t*>(rec + 3 * W);  // zeroed with the records
+        ma.R = R;
+        ma.sst = 2 * R;
         ma.cand = bw.cand.as<uint32_t>();
         ma.dist = bw.planes.as<uint32_t>();
         ma.ctl = ctl;

@@ -16,9 +16,13 @@ struct MsArgs {
     const uint32_t* vnc;
     const uint32_t* qin;   // frontier vertices
     const uint32_t* sin;   // their virtual-node prefix
-    uint64_t* seen;        // n x W
-    uint64_t* visit;       // n x W: bits a frontier vertex pushes this level
-    uint64_t* next;        // n x W: bits received this level
+    // masks of vertex u: seen[u*R + w], visit[u*R + w], next[u*R + w] for w < W; stamp of u:
+    // stamp[u*sst].  Separate arrays have R = W, sst = 1; the one-shot batched pass interleaves
+    // them into one record per vertex (R = 4W: seen, next, visit, stamp), so an edge's mask
+    // read, atomic and stamp exchange touch one cache line of the destination
+    uint64_t* seen;
+    uint64_t* visit;       // bits a frontier vertex pushes this level
+    uint64_t* next;        // bits received this level
     uint32_t* stamp;
     uint32_t* cand;        // vertices that received bits (finalize list)
     uint32_t* qout;        // next frontier
@@ -28,6 +32,7 @@ struct MsArgs {
     uint64_t* act;         // optional: 4 slots x W words; slot level%4 gets the columns that
                            // still have frontier work after this level
     uint32_t W, B, n;
+    uint32_t R, sst;       // mask record stride (64-bit words), stamp stride (32-bit words)
     uint32_t qcap;         // capacity of qout / cand
     uint32_t launch;       // launch index (slot rotation)
     uint32_t level;        // level being discovered by this push (1-based)

@@ -252,6 +252,8 @@ eeg_status Engine::launch_iteration(eeg_graph* g, uint32_t j) {
         a.W = W;
         a.B = B;
         a.n = n;
+        a.R = W;
+        a.sst = 1;
         a.qcap = qcap;
         a.level = j;
         a.stampv = base + j;
