@@ -52,6 +52,7 @@ struct PullArgs {
     uint32_t it;
     uint32_t pers;           // personalised: restart vertex (internal id), else 0xffffffff
     float pers_base;         // its teleport term, 1-d
+    uint32_t strided;        // in place: workgroup b takes tiles b, b + grid, ... (see k_pr_pull)
 };
 
 // New rank of v from its gathered sum; xo = old rank (hi + lo), d = out-degree.
@@ -92,9 +93,15 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
     const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
     const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
     const int grp = threadIdx.x / K, lj = threadIdx.x % K;
-    if (c0 < c1) {
-        uint32_t e_start = wg_search(a.vscan, nE, c0 * TV);
-        for (uint32_t tile = c0; tile < c1; ++tile) {
+    // Tiles: a contiguous run per workgroup, or (in place) strided over the grid, so the first
+    // tiles of every workgroup - the highest-degree vertices in degree order, whose
+    // contributions most edges gather - are finished in the first round and the later rounds
+    // already read their new values (Gauss-Seidel by rounds of `grid` tiles).
+    const uint32_t tb = a.strided ? blockIdx.x : c0, te = a.strided ? ntiles : c1;
+    const uint32_t tstep = a.strided ? gridDim.x : 1u;
+    if (tb < te) {
+        uint32_t e_start = wg_search(a.vscan, nE, tb * TV);
+        for (uint32_t tile = tb; tile < te; tile += tstep) {
             const uint32_t t0 = tile * TV;
             const uint32_t t1 = min(t0 + (uint32_t)TV, T);
             for (int k = threadIdx.x; k < EMAX; k += WG) {
@@ -186,7 +193,8 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                 }
             }
             __syncthreads();
-            e_start += (uint32_t)(nnext - 1);
+            if (!a.strided) e_start += (uint32_t)(nnext - 1);
+            else if (tile + tstep < te) e_start = wg_search(a.vscan, nE, (tile + tstep) * TV);
         }
     }
     // fixed-order block reduction of the FP64 residual
@@ -420,6 +428,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     pa.damping = d;
     pa.pers = pers;
     pa.pers_base = 1.0f - d;
+    pa.strided = inplace ? 1u : 0u;
     double* red = w.red.as<double>();
     auto pull_fn = (K == 4) ? k_pr_pull<4> : (K == 8) ? k_pr_pull<8> : (K == 16) ? k_pr_pull<16> : k_pr_pull<32>;
 
