@@ -835,7 +835,7 @@ __global__ void k_set_frontier(const uint32_t* __restrict__ fv, const uint32_t* 
 __global__ __launch_bounds__(WG) void k_admit(Ctl* ctl, int in_slot, const uint32_t* __restrict__ pv,
                                               const uint32_t* __restrict__ pc, uint32_t npairs,
                                               const uint32_t* __restrict__ uniq, uint32_t nuniq,
-                                              const uint32_t* __restrict__ vnc, uint32_t W, uint32_t B,
+                                              const uint32_t* __restrict__ vnc, uint32_t R, uint32_t B,
                                               uint64_t* m_always, uint64_t* m_edges, uint32_t* lab, uint32_t val,
                                               uint32_t* __restrict__ q, uint32_t* __restrict__ s, uint32_t qcap) {
     __shared__ uint64_t sScan[WG / WAVE + 1];
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(WG) void k_admit(Ctl* ctl, int in_slot, const uint3
     for (uint32_t i = threadIdx.x; i < npairs; i += WG) {
         const uint32_t v = pv[i], c = pc[i];
         lab[(uint64_t)v * B + c] = val;
-        const uint64_t idx = (uint64_t)v * W + c / 64;
+        const uint64_t idx = (uint64_t)v * R + c / 64;
         const unsigned long long bit = 1ull << (c % 64);
         if (m_always) atomicOr((unsigned long long*)&m_always[idx], bit);
         if (m_edges && vnc[v]) atomicOr((unsigned long long*)&m_edges[idx], bit);
@@ -879,15 +879,16 @@ __global__ __launch_bounds__(WG) void k_admit(Ctl* ctl, int in_slot, const uint3
 }
 
 // Frees columns: their label entries go back to `fill` and their bits are cleared.
-__global__ void k_release(uint32_t n, uint32_t W, uint32_t B, uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
-                          uint32_t* __restrict__ lab, uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2) {
+__global__ void k_release(uint32_t n, uint32_t W, uint32_t R, uint32_t B, uint64_t c0, uint64_t c1, uint64_t c2,
+                          uint64_t c3, uint32_t* __restrict__ lab, uint32_t fill, uint64_t* m0, uint64_t* m1,
+                          uint64_t* m2) {
     const uint64_t clr[MAXW] = {c0, c1, c2, c3};
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
         for (uint32_t w = 0; w < W; ++w) {
             uint64_t bits = clr[w];
             if (!bits) continue;
-            const uint64_t idx = v * W + w;
+            const uint64_t idx = v * R + w;
             if (m0) m0[idx] &= ~bits;
             if (m1) m1[idx] &= ~bits;
             if (m2) m2[idx] &= ~bits;
@@ -1300,18 +1301,18 @@ eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStr
 }
 
 eeg_status launch_admit(Ctl* ctl, int in_slot, const uint32_t* pv, const uint32_t* pc, uint32_t npairs,
-                        const uint32_t* uniq, uint32_t nuniq, const uint32_t* vnc, uint32_t W, uint32_t B,
+                        const uint32_t* uniq, uint32_t nuniq, const uint32_t* vnc, uint32_t R, uint32_t B,
                         uint64_t* m_always, uint64_t* m_edges, uint32_t* lab, uint32_t val, uint32_t* q,
                         uint32_t* s, uint32_t qcap, hipStream_t st) {
-    k_admit<<<1, WG, 0, st>>>(ctl, in_slot, pv, pc, npairs, uniq, nuniq, vnc, W, B, m_always, m_edges, lab, val,
+    k_admit<<<1, WG, 0, st>>>(ctl, in_slot, pv, pc, npairs, uniq, nuniq, vnc, R, B, m_always, m_edges, lab, val,
                               q, s, qcap);
     EEG_HIP(hipGetLastError());
     return EEG_OK;
 }
 
-eeg_status launch_release(uint32_t n, uint32_t W, uint32_t B, const uint64_t clr[BATCH_MAXW], uint32_t* lab,
-                          uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2, hipStream_t st) {
-    k_release<<<grid_n(n, 65536), WG, 0, st>>>(n, W, B, clr[0], W > 1 ? clr[1] : 0, W > 2 ? clr[2] : 0,
+eeg_status launch_release(uint32_t n, uint32_t W, uint32_t R, uint32_t B, const uint64_t clr[BATCH_MAXW],
+                          uint32_t* lab, uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2, hipStream_t st) {
+    k_release<<<grid_n(n, 65536), WG, 0, st>>>(n, W, R, B, clr[0], W > 1 ? clr[1] : 0, W > 2 ? clr[2] : 0,
                                                 W > 3 ? clr[3] : 0, lab, fill, m0, m1, m2);
     EEG_HIP(hipGetLastError());
     return EEG_OK;

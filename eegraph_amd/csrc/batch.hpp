@@ -78,17 +78,17 @@ eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStr
 
 // Admission between iterations (one workgroup): pairs (pv[i], pc[i]) = (internal source,
 // column) get label `val`; their column bit is ORed into m_always, and into m_edges when the
-// source has out-edges.  The distinct sources with out-edges (uniq) are appended to the
+// source has out-edges (mask word w of vertex v at index v*R + w).  The distinct sources with out-edges (uniq) are appended to the
 // frontier the next push reads (Ctl slot in_slot, queue q/s of capacity qcap).  A drained
 // engine (Ctl done) restarts from empty slots.
 eeg_status launch_admit(Ctl* ctl, int in_slot, const uint32_t* pv, const uint32_t* pc, uint32_t npairs,
-                        const uint32_t* uniq, uint32_t nuniq, const uint32_t* vnc, uint32_t W, uint32_t B,
+                        const uint32_t* uniq, uint32_t nuniq, const uint32_t* vnc, uint32_t R, uint32_t B,
                         uint64_t* m_always, uint64_t* m_edges, uint32_t* lab, uint32_t val, uint32_t* q,
                         uint32_t* s, uint32_t qcap, hipStream_t st);
 // Frees the columns whose bits are set in clr[0..W): label column back to `fill`, bits cleared
-// in each non-null mask array.
-eeg_status launch_release(uint32_t n, uint32_t W, uint32_t B, const uint64_t clr[BATCH_MAXW], uint32_t* lab,
-                          uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2, hipStream_t st);
+// in each non-null mask array (word w of vertex v at index v*R + w).
+eeg_status launch_release(uint32_t n, uint32_t W, uint32_t R, uint32_t B, const uint64_t clr[BATCH_MAXW],
+                          uint32_t* lab, uint32_t fill, uint64_t* m0, uint64_t* m1, uint64_t* m2, hipStream_t st);
 // out[u] = lab[perm[u] * B + c] (minus `offset` for finite values), original vertex order.
 eeg_status launch_extract(const uint32_t* lab, uint32_t B, uint32_t c, const uint32_t* perm, uint32_t n,
                           uint32_t offset, uint32_t* out, hipStream_t st);

@@ -98,6 +98,7 @@ struct Engine {
     int kind = E_BFS;
     uint32_t n = 0, K = 8;
     uint32_t ncols = 0, W = 0, B = 0, qcap = 0;
+    uint32_t R = 0;  // mask record stride: BFS 4W (seen, next, visit, stamp per vertex), planes W
     hipStream_t st = nullptr;
     std::thread th;
     std::condition_variable cv;
@@ -107,7 +108,7 @@ struct Engine {
     uint32_t running = 0;             // columns in T_RUNNING
     bool broken = false;
     // device state
-    DevMem planes, m1, m2, m3, q[2], s[2], cand, stamp, ctlbuf, stage, spill;
+    DevMem planes, m1, m3, q[2], s[2], cand, stamp, ctlbuf, stage, spill;
     std::vector<std::unique_ptr<DevMem>> res_all;  // result pool (at most ncols buffers)
     std::vector<DevMem*> res_free;                 // guarded by the server mutex
     EngCtl* hctl = nullptr;           // pinned, 4 slots
@@ -201,15 +202,17 @@ eeg_status Engine::init(eeg_graph* g) {
     for (auto& e : ev) EEG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const uint64_t nb = (uint64_t)n * B, nw = (uint64_t)n * W;
     EEG_TRY(planes.alloc(nb * 4 + 4));
-    EEG_TRY(m1.alloc(nw * 8 + 8));
-    EEG_TRY(m3.alloc(nw * 8 + 8));
-    if (kind == E_BFS) EEG_TRY(m2.alloc(nw * 8 + 8));
+    R = (kind == E_BFS) ? 4 * W : W;
+    EEG_TRY(m1.alloc((uint64_t)n * R * 8 + 8));
+    if (kind != E_BFS) {
+        EEG_TRY(m3.alloc(nw * 8 + 8));
+        EEG_TRY(stamp.alloc((size_t)n * 4 + 4));
+    }
     for (int i = 0; i < 2; ++i) {
         EEG_TRY(q[i].alloc((size_t)qcap * 4));
         EEG_TRY(s[i].alloc((size_t)qcap * 4));
     }
     if (kind == E_BFS) EEG_TRY(cand.alloc((size_t)qcap * 4));
-    EEG_TRY(stamp.alloc((size_t)n * 4 + 4));
     EEG_TRY(spill.alloc((size_t)n * 4 + 4));
     EEG_TRY(ctlbuf.alloc(sizeof(EngCtl)));
     stage_words = 3 * (size_t)ncols + 4;
@@ -219,9 +222,8 @@ eeg_status Engine::init(eeg_graph* g) {
     k_eng_fill32<<<grid_sv(nb), WG, 0, st>>>(planes.as<uint32_t>(), nb, fill());
     EEG_HIP(hipGetLastError());
     EEG_HIP(hipMemsetAsync(m1.p, 0, m1.bytes, st));
-    EEG_HIP(hipMemsetAsync(m3.p, 0, m3.bytes, st));
-    if (m2.p) EEG_HIP(hipMemsetAsync(m2.p, 0, m2.bytes, st));
-    EEG_HIP(hipMemsetAsync(stamp.p, 0, stamp.bytes, st));
+    if (m3.p) EEG_HIP(hipMemsetAsync(m3.p, 0, m3.bytes, st));
+    if (stamp.p) EEG_HIP(hipMemsetAsync(stamp.p, 0, stamp.bytes, st));
     EEG_HIP(hipMemsetAsync(ctlbuf.p, 0, sizeof(EngCtl), st));
     // a drained engine: the first admission starts from empty frontier slots
     k_eng_fill32<<<1, 64, 0, st>>>(&ctlbuf.as<EngCtl>()->ctl.done, 1, 1u);
@@ -241,10 +243,11 @@ eeg_status Engine::launch_iteration(eeg_graph* g, uint32_t j) {
         a.row = g->out.rowp();
         a.col = g->out.colp();
         a.vnc = t.vnc.as<uint32_t>();
-        a.seen = m1.as<uint64_t>();
-        a.visit = m2.as<uint64_t>();
-        a.next = m3.as<uint64_t>();
-        a.stamp = stamp.as<uint32_t>();
+        uint64_t* rec = m1.as<uint64_t>();
+        a.seen = rec;
+        a.next = rec + W;
+        a.visit = rec + 2 * W;
+        a.stamp = reinterpret_cast<uint32_t*>(rec + 3 * W);
         a.cand = cand.as<uint32_t>();
         a.dist = planes.as<uint32_t>();
         a.ctl = &ec->ctl;
@@ -252,8 +255,8 @@ eeg_status Engine::launch_iteration(eeg_graph* g, uint32_t j) {
         a.W = W;
         a.B = B;
         a.n = n;
-        a.R = W;
-        a.sst = 1;
+        a.R = R;
+        a.sst = 2 * R;
         a.qcap = qcap;
         a.level = j;
         a.stampv = base + j;
@@ -369,9 +372,10 @@ void Engine::loop() {
         if (any_clr) {
             // a converged plane column has no change bits left; a BFS column keeps its seen
             // bits and visit bits on vertices without out-edges
-            rc = launch_release(n, W, B, clr, planes.as<uint32_t>(), fill(),
-                                kind == E_BFS ? m1.as<uint64_t>() : nullptr,
-                                kind == E_BFS ? m2.as<uint64_t>() : nullptr, nullptr, st);
+            rc = launch_release(n, W, R, B, clr, planes.as<uint32_t>(), fill(),
+                                kind == E_BFS ? m1.as<uint64_t>() : nullptr,                  // seen
+                                kind == E_BFS ? m1.as<uint64_t>() + 2 * W : nullptr, nullptr,  // visit
+                                st);
         }
         if (rc == EEG_OK && !pv.empty()) {
             uint32_t* hs = hstage + (size_t)(it % 4) * stage_words;
@@ -389,8 +393,8 @@ void Engine::loop() {
             const int cur = (int)((it - 1) & 1);
             if (rc == EEG_OK)
                 rc = launch_admit(&ec->ctl, in_slot, ds, ds + np, np, ds + 2 * np, (uint32_t)uniq.size(),
-                                  t.vnc.as<uint32_t>(), W, B, kind == E_BFS ? m1.as<uint64_t>() : nullptr,
-                                  kind == E_BFS ? m2.as<uint64_t>() : m1.as<uint64_t>(), planes.as<uint32_t>(),
+                                  t.vnc.as<uint32_t>(), R, B, kind == E_BFS ? m1.as<uint64_t>() : nullptr,
+                                  kind == E_BFS ? m1.as<uint64_t>() + 2 * W : m1.as<uint64_t>(), planes.as<uint32_t>(),
                                   kind == E_BFS ? it - 1 : seed_val(),  // BFS: levels are global
                                   q[cur].as<uint32_t>(), s[cur].as<uint32_t>(), qcap, st);
         }
