@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--serve-scale", type=int, default=26)
     p.add_argument("--serve-qps", type=float, default=100.0)
     p.add_argument("--serve-seconds", type=float, default=3.0)
+    p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp"),
+                   help="only run two multi-source passes (the second timed) of this algorithm on the "
+                        "batched section's graph and sources, for profiling the batched kernels")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 512 MiB device-to-device copy kernel three times "
                         "(scale reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -206,6 +209,24 @@ def batched_section(eg, a):
     return res
 
 
+def batched_only(eg, a):
+    """Two multi-source passes of one algorithm (RMAT scale --batch-scale, EF 16, seed 4, the
+    same 256 sources as the batched section); prints the timed pass as one JSON line."""
+    import numpy as np
+
+    g = eg.Graph.rmat(a.batch_scale, 16, seed=4)
+    n = g.info()["n"]
+    srcs = np.random.default_rng(4).integers(0, n, 256).astype(np.uint32)
+    g.run_batch(a.batched_only, srcs, want_labels=False)
+    t = time.perf_counter()
+    _, st = g.run_batch(a.batched_only, srcs, want_labels=False)
+    bat = time.perf_counter() - t
+    print(json.dumps({"batched_only": a.batched_only, "workload": f"rmat{a.batch_scale}_ef16_seed4_256_sources",
+                      "ms": round(bat * 1e3, 3), "ms_traversal": round(st["ms_traversal"], 3),
+                      "queries_per_s": round(256 / bat, 1), "iterations": st["iterations"]}), flush=True)
+    g.close()
+
+
 def grid_section(eg, a):
     """High-diameter case: 4096 x 4096 grid, 4-neighbour edges both ways plus 1% random
     shortcut pairs (seed 3), w ~ U{1..1000}, source 0.  BFS/SSSP/SSWP/CC with compacted
@@ -328,6 +349,10 @@ def main():
 
     import eegraph_amd as eg
     from eegraph_amd import _lib as L
+
+    if a.batched_only:
+        batched_only(eg, a)
+        return
 
     algs = [x for x in a.algs.split(",") if x]
     variants = [VARIANT_OF[v] for v in a.variants.split(",") if v]
