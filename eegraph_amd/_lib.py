@@ -51,7 +51,8 @@ class Opts(C.Structure):
     _fields_ = [("variant", u32), ("K", u32), ("block", u32), ("frontier", u32),
                 ("max_iters", u32), ("collect_activity", u32), ("timing", u32),
                 ("count_work", u32), ("damping", C.c_float), ("acc", C.c_float),
-                ("epsilon", C.c_double), ("wgs_per_cu", u32), ("reserved", u32 * 7)]
+                ("epsilon", C.c_double), ("wgs_per_cu", u32), ("energy", u32),
+                ("reserved", u32 * 6)]
 
 
 class Stats(C.Structure):

@@ -78,7 +78,7 @@ def save_edgelist(path, n, src, dst, w=None):
 
 def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timing=False,
               count_work=False, collect_activity=False, damping=0.85, epsilon=0.0, acc=0.0,
-              wgs_per_cu=0):
+              wgs_per_cu=0, energy=False):
     o = L.Opts()
     o.variant = L.VARIANTS[variant] if isinstance(variant, str) else int(variant)
     o.K = K
@@ -93,6 +93,7 @@ def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timi
     o.epsilon = epsilon
     o.acc = acc
     o.wgs_per_cu = wgs_per_cu
+    o.energy = 1 if energy else 0
     return o
 
 

@@ -291,7 +291,7 @@ static eeg_status run_common(eeg_graph* g, uint32_t algo, uint32_t source, const
     EEG_HIP(hipSetDevice(g->device));
     const auto t0 = std::chrono::steady_clock::now();
     double e0 = 0, e1 = 0;
-    const bool energy = eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
+    const bool energy = o.energy && eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
     const uint32_t isrc = (sourced && g->n) ? g->h_perm[source] : 0;  // internal id
     eeg_status s = (algo == EEG_PR || algo == EEG_PPR) ? run_pagerank(g, algo, isrc, o, st)
                                                         : run_traversal(g, algo, isrc, o, st);
@@ -348,7 +348,7 @@ extern "C" eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t*
     EEG_HIP(hipSetDevice(g->device));
     const auto t0 = std::chrono::steady_clock::now();
     double e0 = 0, e1 = 0;
-    const bool energy = eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
+    const bool energy = o.energy && eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
     if (algo == EEG_PPR) {
         // one pull iteration per source: planes of several restart vertices were measured
         // slower (the contribution plane outgrows the caches the single gather relies on)

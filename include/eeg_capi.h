@@ -63,7 +63,7 @@ typedef enum eeg_algo {
  *                   form the next frontier (compacted queue, or per-vertex flags).
  * For PR the sync variants run a Jacobi iteration with destination-owned compensated sums
  * (deterministic); the async variants run the same gather updating ranks in place.
- * Personalised PageRank runs residual (delta) push from the source. */
+ * Personalised PageRank runs the same gather with a one-hot restart at the source. */
 typedef enum eeg_variant {
     EEG_ASYNC_PUSH_TD = 0,
     EEG_ASYNC_PUSH_DD = 1,
@@ -93,7 +93,9 @@ typedef struct eeg_opts {
     double epsilon;            /* PR and PPR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7) */
     uint32_t wgs_per_cu;       /* resident workgroups per compute unit of the tile kernels
                                   (0 = default: 24 for sweeps, 16 for frontier queues and the PageRank gather) */
-    uint32_t reserved[7];
+    uint32_t energy;           /* read the amd-smi energy counter before and after the run into
+                                  stats->energy_j (two counter reads, ~0.25 ms each; 0 = off) */
+    uint32_t reserved[6];
 } eeg_opts;
 
 typedef struct eeg_stats {
@@ -110,7 +112,7 @@ typedef struct eeg_stats {
     uint64_t label_updates;     /* successful atomic label improvements */
     double bytes_alg;           /* algorithmic HBM bytes of the traversal */
     double pr_residual;         /* PR: final L1 residual (FP64); PPR: residual mass */
-    double energy_j;            /* device energy over the traversal (0 if unavailable) */
+    double energy_j;            /* device energy over the run with opts.energy (else 0, or if unavailable) */
     uint32_t* activity;         /* optional caller buffer for per-iteration frontier sizes */
     uint32_t activity_cap;      /* capacity of activity */
     uint32_t reserved[7];

@@ -146,6 +146,17 @@ def test_work_counters_and_timing(rmat14, refs14):
     assert st["kernels_timed"] == st["iterations"] and st["ms_kernel_sum"] > 0
 
 
+def test_energy_reading_is_opt_in(rmat14, refs14):
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run("bfs", src, variant="sync_push_dd")
+    assert np.array_equal(lab, refs["bfs"]) and st["energy_j"] == 0.0
+    lab, st = g.run("bfs", src, variant="sync_push_dd", energy=True)
+    assert np.array_equal(lab, refs["bfs"]) and st["energy_j"] >= 0.0
+    if eg.energy_read(0) is None:
+        assert st["energy_j"] == 0.0
+
+
 def _rows_sorted(row, col):
     out = col.copy()
     for v in range(len(row) - 1):
