@@ -169,7 +169,7 @@ extern "C" eeg_status eeg_run_packed(eeg_graph* const* graphs, uint32_t ngraphs,
     hipStream_t s = u->stream;
 
     double e0 = 0, e1 = 0;
-    const bool energy = eeg_energy_read(u->device, &e0, nullptr) == EEG_OK;
+    const bool energy = o.energy && eeg_energy_read(u->device, &e0, nullptr) == EEG_OK;
     eeg_status rc;
     if (sourced) {
         std::vector<uint32_t> seeds(ngraphs);
