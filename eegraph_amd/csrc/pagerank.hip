@@ -145,31 +145,33 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
             // one in-edge per lane, K lanes per virtual node, TwoSum tree per virtual node;
             // the source indices of all rounds are loaded first, then all contributions, so a
             // lane keeps ROUNDS independent requests in flight
+            // Branch-free rounds: a slot past the tile or past its virtual node's edges reads
+            // edge 0 and its contribution (always in range, and one shared address per wave)
+            // and drops the value, so no exec-mask branches surround the loads.
             uint32_t pidx[ROUNDS];
             bool pv[ROUNDS];
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
-                const int slot = r * VPR + grp;
+                const int slot = r * VPR + grp;  // < TV: sOwn[slot] is in range
                 const uint32_t t = t0 + slot;
-                pv[r] = false;
-                pidx[r] = 0;
-                if (t < t1) {
-                    const int k = sOwn[slot];
-                    const uint32_t eoff = (t - sS[k]) * K + lj;
-                    if (eoff < sDeg[k]) {
-                        pv[r] = true;
-                        // streamed once per iteration: keep it from evicting the gathered contributions
-                        pidx[r] = __builtin_nontemporal_load(&a.col[sRow[k] + eoff]);
-                    }
-                }
+                const bool tin = t < t1;
+                const int k = tin ? (int)sOwn[slot] : 0;
+                const uint32_t eoff = (t - sS[k]) * K + lj;
+                pv[r] = tin && eoff < sDeg[k];
+                const uint64_t e = pv[r] ? sRow[k] + eoff : 0ull;
+                // streamed once per iteration: keep it from evicting the gathered contributions
+                pidx[r] = __builtin_nontemporal_load(&a.col[e]);
             }
             float pc[ROUNDS];
 #pragma unroll
-            for (int r = 0; r < ROUNDS; ++r) pc[r] = pv[r] ? a.c_old[pidx[r]] : 0.0f;
+            for (int r = 0; r < ROUNDS; ++r) {
+                const float c = a.c_old[pidx[r]];
+                pc[r] = pv[r] ? c : 0.0f;
+            }
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
                 const int slot = r * VPR + grp;
-                const Comp g = group_reduce<K>(Comp{pc[r], 0.0f});
+                const Comp g = group_sum_first_lane<K>(pc[r]);
                 if (lj == 0 && t0 + slot < t1) sPart[slot] = make_float2(g.s, g.e);
             }
             __syncthreads();
