@@ -178,6 +178,31 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
             // atomic, so a tile keeps ROUNDS independent requests in flight per lane.
             uint32_t ed[ROUNDS], ew[ROUNDS], el[ROUNDS];
             bool ev[ROUNDS];
+            // Unweighted frontier queues (BFS, CC) issue the edge and label loads from every
+            // lane without exec-mask branches: a slot past the tile or past its virtual node's
+            // edges reads edge 0 and that edge's destination label (in range, one shared
+            // address per wave) and ev[] drops them.  Measured faster there; the sweeps (many
+            // inactive virtual nodes) and the weighted queues keep the branches.
+            constexpr bool kFlat = (MODE == M_QUEUE) && !R::weighted;
+            if constexpr (kFlat) {
+#pragma unroll
+                for (int r = 0; r < ROUNDS; ++r) {
+                    const int slot = r * VPR + grp;  // < TV: sOwn[slot] is in range
+                    const uint32_t t = t0 + slot;
+                    const bool tin = t < t1;
+                    const int k = tin ? (int)sOwn[slot] : 0;
+                    const uint32_t eoff = (t - sS[k]) * K + lj;
+                    const uint32_t deg = sDeg[k];
+                    const bool act = tin && deg > 0;
+                    if (act && lj == 0) ++st_vn;
+                    ev[r] = act && eoff < deg;
+                    const uint64_t e = ev[r] ? sRow[k] + eoff : 0ull;
+                    ed[r] = __builtin_nontemporal_load(&a.col[e]);
+                    ew[r] = 1u;
+                    el[r] = sLab[k];
+                    st_edge += ev[r];
+                }
+            } else {
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
                 const int slot = r * VPR + grp;
@@ -208,13 +233,19 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                     }
                 }
             }
+            }
             bool eu[ROUNDS];
             {
                 uint32_t ecur[ROUNDS];
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     ew[r] = R::cand(el[r], ew[r]);  // label offered to the destination
-                    ecur[r] = ev[r] ? a.lab[ed[r]] : 0u;
+                    if constexpr (kFlat) {
+                        const uint32_t cur = a.lab[ed[r]];
+                        ecur[r] = ev[r] ? cur : 0u;
+                    } else {
+                        ecur[r] = ev[r] ? a.lab[ed[r]] : 0u;
+                    }
                 }
                 uint32_t eo[ROUNDS];
                 bool ea[ROUNDS];
