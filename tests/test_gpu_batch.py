@@ -20,6 +20,29 @@ def rmat13():
     g.close()
 
 
+@pytest.fixture(scope="module")
+def rmat22():
+    g = eg.Graph.rmat(22, 16, seed=2, wmin=1, wmax=64)
+    yield g
+    g.close()
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+def test_batch_full_width_rmat22(rmat22, alg):
+    """256 sources (four 64-bit mask words per vertex) on RMAT scale 22, EF 16: columns from
+    every mask word equal their single-query runs bit for bit."""
+    g = rmat22
+    n = g.info()["n"]
+    srcs = np.random.default_rng(22).integers(0, n, 256).astype(np.uint32)
+    srcs[0] = g.info()["argmax_out_degree"]
+    out, st = g.run_batch(alg, srcs, K=8)
+    assert out.shape == (256, n)
+    for q in (0, 1, 63, 64, 127, 128, 200, 255):
+        single, _ = g.run(alg, int(srcs[q]), variant="sync_push_dd", K=8)
+        assert np.array_equal(out[q], single), (alg, q)
+    del out
+
+
 @pytest.mark.parametrize("alg", list(ALG))
 @pytest.mark.parametrize("nsrc", [1, 37, 64, 100, 150, 200, 256, 300])
 def test_batch_equals_single_and_oracle(rmat13, alg, nsrc):
