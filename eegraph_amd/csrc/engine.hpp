@@ -79,6 +79,7 @@ struct Tigr {
     DevMem vnc;    // u32[n]: ceil(deg/K) per vertex (virtual nodes a frontier entry brings)
     bool built = false;
     DevMem span;        // in-edge tables only: positions of vertices crossing a PageRank tile
+    DevMem tstart;      // in-edge tables only: first entry (vlist position) of each PageRank tile
     uint32_t nspan = 0;
     bool span_built = false;
 };

@@ -53,6 +53,7 @@ struct PullArgs {
     uint32_t pers;           // personalised: restart vertex (internal id), else 0xffffffff
     float pers_base;         // its teleport term, 1-d
     uint32_t strided;        // in place: workgroup b takes tiles b, b + grid, ... (see k_pr_pull)
+    const uint32_t* tstart;  // first entry of each tile (vlist position)
 };
 
 // New rank of v from its gathered sum; xo = old rank (hi + lo), d = out-degree.
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
     const uint32_t tb = a.strided ? blockIdx.x : c0, te = a.strided ? ntiles : c1;
     const uint32_t tstep = a.strided ? gridDim.x : 1u;
     if (tb < te) {
-        uint32_t e_start = wg_search(a.vscan, nE, tb * TV);
+        uint32_t e_start = a.tstart[tb];
         for (uint32_t tile = tb; tile < te; tile += tstep) {
             const uint32_t t0 = tile * TV;
             const uint32_t t1 = min(t0 + (uint32_t)TV, T);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
             }
             __syncthreads();
             if (!a.strided) e_start += (uint32_t)(nnext - 1);
-            else if (tile + tstep < te) e_start = wg_search(a.vscan, nE, (tile + tstep) * TV);
+            else if (tile + tstep < te) e_start = a.tstart[tile + tstep];
         }
     }
     // fixed-order block reduction of the FP64 residual
@@ -315,6 +316,20 @@ __global__ void k_outdeg(const uint64_t* __restrict__ row, uint32_t n, uint32_t*
         deg[v] = (uint32_t)(row[v + 1] - row[v]);
 }
 
+// First entry of every tile: the largest i < nz with vscan[i] <= tile * TV.
+__global__ void k_tile_starts(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV, uint32_t ntiles,
+                              uint32_t* __restrict__ tstart) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        const uint64_t first = (uint64_t)t * TV;
+        uint32_t lo = 0, hi = nz;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (vscan[mid] <= first) lo = mid; else hi = mid;
+        }
+        tstart[t] = lo;
+    }
+}
+
 __global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV,
                                 uint32_t* __restrict__ span, uint32_t* __restrict__ count) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nz;
@@ -347,9 +362,13 @@ eeg_status ensure_pr_span(eeg_graph* g, uint32_t K) {
     EEG_TRY(t.span.alloc((size_t)ntiles * 4 + 4));
     EEG_TRY(cnt.alloc(4));
     EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    EEG_TRY(t.tstart.alloc((size_t)ntiles * 4 + 4));
     if (t.nz) {
         k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
             t.vscan.as<uint32_t>(), t.nz, TV, t.span.as<uint32_t>(), cnt.as<uint32_t>());
+        EEG_HIP(hipGetLastError());
+        k_tile_starts<<<std::max<uint64_t>(1, std::min<uint64_t>(4096, (ntiles + WG - 1) / WG)), WG, 0, s>>>(
+            t.vscan.as<uint32_t>(), t.nz, TV, (uint32_t)ntiles, t.tstart.as<uint32_t>());
         EEG_HIP(hipGetLastError());
     }
     uint32_t nspan = 0;
@@ -431,6 +450,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     pa.pers = pers;
     pa.pers_base = 1.0f - d;
     pa.strided = inplace ? 1u : 0u;
+    pa.tstart = t.tstart.as<uint32_t>();
     double* red = w.red.as<double>();
     auto pull_fn = (K == 4) ? k_pr_pull<4> : (K == 8) ? k_pr_pull<8> : (K == 16) ? k_pr_pull<16> : k_pr_pull<32>;
 
