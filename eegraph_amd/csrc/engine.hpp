@@ -78,8 +78,11 @@ struct Tigr {
     DevMem vscan;  // u32[nz+1]
     DevMem vnc;    // u32[n]: ceil(deg/K) per vertex (virtual nodes a frontier entry brings)
     bool built = false;
+    DevMem tstart;      // first entry (vlist position) of every traversal tile of the static sweep
+    DevMem tblk;        // per tile: first and last 64-vertex block of its entries (lo | hi << 32)
+    bool tiles_built = false;
     DevMem span;        // in-edge tables only: positions of vertices crossing a PageRank tile
-    DevMem tstart;      // in-edge tables only: first entry (vlist position) of each PageRank tile
+    DevMem prstart;     // in-edge tables only: first entry (vlist position) of each PageRank tile
     uint32_t nspan = 0;
     bool span_built = false;
 };
@@ -92,6 +95,7 @@ struct Work {
     DevMem stamp;       // u32[n] iteration stamps for queue dedup
     DevMem flag;        // u32[n] iteration stamps for flag-scan frontiers
     DevMem vnpush;      // u32[P] last label pushed by each virtual node (async dd)
+    DevMem vdirty;      // u32 per 64-vertex block: stamp of the last improvement in the block
     DevMem ctl;         // Ctl
     DevMem fx[2], fl[2], fc[2];  // PR: ranks as (hi, lo) pairs and contributions x/outdeg
     DevMem spill;       // PR: per-tile partial sums

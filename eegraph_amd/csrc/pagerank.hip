@@ -362,13 +362,13 @@ eeg_status ensure_pr_span(eeg_graph* g, uint32_t K) {
     EEG_TRY(t.span.alloc((size_t)ntiles * 4 + 4));
     EEG_TRY(cnt.alloc(4));
     EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
-    EEG_TRY(t.tstart.alloc((size_t)ntiles * 4 + 4));
+    EEG_TRY(t.prstart.alloc((size_t)ntiles * 4 + 4));
     if (t.nz) {
         k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
             t.vscan.as<uint32_t>(), t.nz, TV, t.span.as<uint32_t>(), cnt.as<uint32_t>());
         EEG_HIP(hipGetLastError());
         k_tile_starts<<<std::max<uint64_t>(1, std::min<uint64_t>(4096, (ntiles + WG - 1) / WG)), WG, 0, s>>>(
-            t.vscan.as<uint32_t>(), t.nz, TV, (uint32_t)ntiles, t.tstart.as<uint32_t>());
+            t.vscan.as<uint32_t>(), t.nz, TV, (uint32_t)ntiles, t.prstart.as<uint32_t>());
         EEG_HIP(hipGetLastError());
     }
     uint32_t nspan = 0;
@@ -450,7 +450,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     pa.pers = pers;
     pa.pers_base = 1.0f - d;
     pa.strided = inplace ? 1u : 0u;
-    pa.tstart = t.tstart.as<uint32_t>();
+    pa.tstart = t.prstart.as<uint32_t>();
     double* red = w.red.as<double>();
     auto pull_fn = (K == 4) ? k_pr_pull<4> : (K == 8) ? k_pr_pull<8> : (K == 16) ? k_pr_pull<16> : k_pr_pull<32>;
 

@@ -57,6 +57,15 @@ struct RelaxArgs {
     // ping-pong are derived from them, so one captured chunk serves every later chunk and run.
     const uint32_t* dev_it;
     uint32_t *qa, *sa, *qb, *sb;  // queue buffers 0 and 1
+    // Tile skipping of the static sweeps (M_ASYNC, M_FLAGS; not CC, whose activity is dense):
+    // an improvement of vertex u in iteration it stamps u's 64-vertex block with tbase + it,
+    // and iteration it runs only the tiles whose entries cover a block stamped in iteration
+    // it-1 or it (stamps only grow, so nothing is cleared; stamps of earlier traversals are
+    // below tbase).  Iteration 1 runs the blocks stamped tbase + 0 at the start (the seeds).
+    const uint32_t* tstart;  // first entry of each tile
+    const uint64_t* tblk;    // first and last block of each tile's entries
+    uint32_t* vdirty;
+    uint32_t tbase;
 };
 
 template <int ALG, int K, int MODE>
@@ -126,9 +135,18 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
     const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
     const int grp = threadIdx.x / K, lj = threadIdx.x % K;
 
+    constexpr bool SKIP = (MODE == M_ASYNC || MODE == M_FLAGS) && ALG != A_CC;
     if (c0 < c1) {
-        uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+        uint32_t e_start = SKIP ? 0u : wg_search(a.sin, nE, c0 * TV);
         for (uint32_t tile = c0; tile < c1; ++tile) {
+            if constexpr (SKIP) {
+                const uint64_t bb = a.tblk[tile];
+                const uint32_t blo = (uint32_t)bb, bhi = (uint32_t)(bb >> 32);
+                bool hot = false;
+                for (uint32_t b = blo + threadIdx.x; b <= bhi; b += WG) hot |= a.vdirty[b] + 1u >= a.tbase + a.it;
+                if (!__syncthreads_or(hot)) continue;  // the same answer in every thread
+                e_start = a.tstart[tile];
+            }
             const uint32_t t0 = tile * TV;
             const uint32_t t1 = min(t0 + (uint32_t)TV, T);
             for (int k = threadIdx.x; k < EMAX; k += WG) {
@@ -297,6 +315,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                     if (upd) {
                         changed = true;
                         if (MODE == M_FLAGS) a.flag[dv] = a.stampv + 1;
+                        if constexpr (SKIP) a.vdirty[dv >> 6] = a.tbase + a.it;
                     }
                 }
             }
@@ -337,7 +356,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                 }
             }
             __syncthreads();
-            e_start += (uint32_t)(nnext - 1);
+            if constexpr (!SKIP) e_start += (uint32_t)(nnext - 1);
         }
     }
     if (MODE != M_QUEUE) {
@@ -370,6 +389,37 @@ __global__ void k_init_labels(uint32_t* __restrict__ lab, uint32_t n, uint32_t f
         const uint32_t x = (v == source) ? at_source : fill;
         lab[v] = ids ? ids[v] : x;  // CC: the original id of internal vertex v
     }
+}
+
+// Tables of the tile skipping: first entry of every tile (the largest i < nz with
+// vscan[i] <= tile * TV) and first virtual node of every vertex with edges.
+__global__ void k_tile_starts(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV, uint32_t ntiles,
+                              uint32_t* __restrict__ tstart) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        const uint64_t first = (uint64_t)t * TV;
+        uint32_t lo = 0, hi = nz;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (vscan[mid] <= first) lo = mid; else hi = mid;
+        }
+        tstart[t] = lo;
+    }
+}
+// Blocks of every tile: from its first entry's vertex to the vertex of the entry that holds
+// the next tile's first virtual node (a superset of the tile's entries).
+__global__ void k_tile_blocks(const uint32_t* __restrict__ vlist, const uint32_t* __restrict__ tstart, uint32_t nz,
+                              uint32_t ntiles, uint64_t* __restrict__ tblk) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        const uint32_t e0 = tstart[t], e1 = (t + 1 < ntiles) ? tstart[t + 1] : nz - 1;
+        tblk[t] = (uint64_t)(vlist[e0] >> 6) | ((uint64_t)(vlist[e1] >> 6) << 32);
+    }
+}
+// Stamps the blocks of the seed vertices (or of `single` when seeds is null) with `val`.
+__global__ void k_mark_blocks(const uint32_t* __restrict__ seeds, uint32_t nseeds, uint32_t single,
+                              uint32_t* __restrict__ vdirty, uint32_t val) {
+    const uint32_t cnt = seeds ? nseeds : 1u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        vdirty[(seeds ? seeds[i] : single) >> 6] = val;
 }
 
 // Advances the device iteration counter of a replayed chunk (one thread, vector store).
@@ -514,6 +564,24 @@ size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
 
 namespace {
 
+// Tile tables of a Tigr layout for the static sweeps (built once per graph and K).
+eeg_status ensure_tiles(Tigr& t, uint32_t K, hipStream_t s) {
+    if (t.tiles_built) return EEG_OK;
+    const uint32_t TV = TILE_EDGES / K;
+    const uint64_t ntiles = (t.P + TV - 1) / TV;
+    EEG_TRY(t.tstart.alloc((size_t)ntiles * 4 + 4));
+    EEG_TRY(t.tblk.alloc((size_t)ntiles * 8 + 8));
+    if (t.nz) {
+        const unsigned gt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(4096, (ntiles + WG - 1) / WG));
+        k_tile_starts<<<gt, WG, 0, s>>>(t.vscan.as<uint32_t>(), t.nz, TV, (uint32_t)ntiles, t.tstart.as<uint32_t>());
+        k_tile_blocks<<<gt, WG, 0, s>>>(t.vlist.as<uint32_t>(), t.tstart.as<uint32_t>(), t.nz, (uint32_t)ntiles,
+                                        t.tblk.as<uint64_t>());
+        EEG_HIP(hipGetLastError());
+    }
+    t.tiles_built = true;
+    return EEG_OK;
+}
+
 // Iterations per replayed chunk: a multiple of 2 (queue ping-pong) and 3 (control slots), so
 // every replay starts in the same state pattern.
 constexpr uint32_t GRAPH_CHUNK = 6;
@@ -628,9 +696,20 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         EEG_TRY(w.flag.alloc((size_t)n * 4 + 4));
         EEG_HIP(hipMemsetAsync(w.flag.p, 0, w.flag.bytes, s));
     }
+    // tile skipping of the static sweeps: tables of the layout, stamps per tile
+    const bool skip = (mode == M_ASYNC || mode == M_FLAGS) && alg != A_CC;
+    if (skip) {
+        EEG_TRY(ensure_tiles(tt[k_index(K)], K, s));
+        const size_t need = (size_t)((n + 63) / 64) * 4 + 4;
+        if (w.vdirty.bytes < need) {
+            EEG_TRY(w.vdirty.alloc(need));
+            EEG_HIP(hipMemsetAsync(w.vdirty.p, 0, w.vdirty.bytes, s));
+        }
+    }
     if ((uint64_t)w.epoch + max_iters + 200 >= 0xfffffff0ull) {  // stamps would wrap: reset
         if (w.stamp.p) EEG_HIP(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, s));
         if (w.flag.p) EEG_HIP(hipMemsetAsync(w.flag.p, 0, w.flag.bytes, s));
+        if (w.vdirty.p) EEG_HIP(hipMemsetAsync(w.vdirty.p, 0, w.vdirty.bytes, s));
         w.epoch = 1;
     }
     if (mode == M_ASYNC) EEG_TRY(w.vnpush.alloc((size_t)t.P * 4 + 4));
@@ -653,6 +732,12 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     }
     const uint32_t base = w.epoch;  // iteration it uses stamp base + it
     const bool seeded = (alg != A_CC);
+    if (skip) {  // the first sweep starts from the seeds' blocks
+        const uint32_t cnt = d_seeds ? nseeds : 1u;
+        k_mark_blocks<<<std::max(1u, std::min(1024u, (cnt + WG - 1) / WG)), WG, 0, s>>>(
+            d_seeds, nseeds, source, w.vdirty.as<uint32_t>(), base);
+        EEG_HIP(hipGetLastError());
+    }
     Ctl* ctl = w.ctl.as<Ctl>();
     k_init_ctl<<<1, 64, 0, s>>>(ctl, c.rowp(), K, (seeded && mode == M_QUEUE && !d_seeds) ? 1 : 0, source,
                                 mode == M_QUEUE ? w.q[0].as<uint32_t>() : nullptr,
@@ -699,6 +784,12 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     ra.n_static = t.nz;
     ra.P_static = (uint32_t)t.P;
     ra.count_stats = (o.count_work || o.collect_activity) ? 1u : 0u;
+    if (skip) {
+        ra.tstart = t.tstart.as<uint32_t>();
+        ra.tblk = t.tblk.as<uint64_t>();
+        ra.vdirty = w.vdirty.as<uint32_t>();
+        ra.tbase = base;
+    }
 
     // Event pool: per launch L (it = L+1) an "after" event at 2L+1 and, with timing, a
     // "before" event at 2L; event 0 of the pool tail marks the traversal start.
