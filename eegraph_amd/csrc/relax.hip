@@ -57,15 +57,17 @@ struct RelaxArgs {
     // ping-pong are derived from them, so one captured chunk serves every later chunk and run.
     const uint32_t* dev_it;
     uint32_t *qa, *sa, *qb, *sb;  // queue buffers 0 and 1
-    // Tile skipping of the static sweeps (M_ASYNC, M_FLAGS; not CC, whose activity is dense):
-    // an improvement of vertex u in iteration it stamps u's 64-vertex block with tbase + it,
-    // and iteration it runs only the tiles whose entries cover a block stamped in iteration
-    // it-1 or it (stamps only grow, so nothing is cleared; stamps of earlier traversals are
-    // below tbase).  Iteration 1 runs the blocks stamped tbase + 0 at the start (the seeds).
+    // Tile skipping of the static sweeps (M_ASYNC; M_FLAGS except CC, whose flag scan measured
+    // slower with it): an improvement of vertex u in iteration it stamps u's 64-vertex block
+    // with tbase + it, and iteration it runs only the tiles whose entries cover a block stamped
+    // in iteration it-1 or it (stamps only grow, so nothing is cleared; stamps of earlier
+    // traversals are below tbase).  Iteration 1 runs the blocks stamped tbase + 0 at the start
+    // (the seeds), or every tile with all_first.
     const uint32_t* tstart;  // first entry of each tile
     const uint64_t* tblk;    // first and last block of each tile's entries
     uint32_t* vdirty;
     uint32_t tbase;
+    uint32_t all_first;      // iteration 1 runs every tile (CC: every vertex starts active)
 };
 
 template <int ALG, int K, int MODE>
@@ -135,15 +137,16 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
     const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
     const int grp = threadIdx.x / K, lj = threadIdx.x % K;
 
-    constexpr bool SKIP = (MODE == M_ASYNC || MODE == M_FLAGS) && ALG != A_CC;
+    constexpr bool SKIP = MODE == M_ASYNC || (MODE == M_FLAGS && ALG != A_CC);
     if (c0 < c1) {
         uint32_t e_start = SKIP ? 0u : wg_search(a.sin, nE, c0 * TV);
         for (uint32_t tile = c0; tile < c1; ++tile) {
             if constexpr (SKIP) {
                 const uint64_t bb = a.tblk[tile];
                 const uint32_t blo = (uint32_t)bb, bhi = (uint32_t)(bb >> 32);
-                bool hot = false;
-                for (uint32_t b = blo + threadIdx.x; b <= bhi; b += WG) hot |= a.vdirty[b] + 1u >= a.tbase + a.it;
+                bool hot = a.it == 1 && a.all_first;
+                if (!hot)
+                    for (uint32_t b = blo + threadIdx.x; b <= bhi; b += WG) hot |= a.vdirty[b] + 1u >= a.tbase + a.it;
                 if (!__syncthreads_or(hot)) continue;  // the same answer in every thread
                 e_start = a.tstart[tile];
             }
@@ -697,7 +700,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         EEG_HIP(hipMemsetAsync(w.flag.p, 0, w.flag.bytes, s));
     }
     // tile skipping of the static sweeps: tables of the layout, stamps per tile
-    const bool skip = (mode == M_ASYNC || mode == M_FLAGS) && alg != A_CC;
+    const bool skip = mode == M_ASYNC || (mode == M_FLAGS && alg != A_CC);
     if (skip) {
         EEG_TRY(ensure_tiles(tt[k_index(K)], K, s));
         const size_t need = (size_t)((n + 63) / 64) * 4 + 4;
@@ -732,7 +735,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     }
     const uint32_t base = w.epoch;  // iteration it uses stamp base + it
     const bool seeded = (alg != A_CC);
-    if (skip) {  // the first sweep starts from the seeds' blocks
+    if (skip && seeded) {  // the first sweep starts from the seeds' blocks
         const uint32_t cnt = d_seeds ? nseeds : 1u;
         k_mark_blocks<<<std::max(1u, std::min(1024u, (cnt + WG - 1) / WG)), WG, 0, s>>>(
             d_seeds, nseeds, source, w.vdirty.as<uint32_t>(), base);
@@ -789,6 +792,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         ra.tblk = t.tblk.as<uint64_t>();
         ra.vdirty = w.vdirty.as<uint32_t>();
         ra.tbase = base;
+        ra.all_first = seeded ? 0u : 1u;
     }
 
     // Event pool: per launch L (it = L+1) an "after" event at 2L+1 and, with timing, a
