@@ -146,6 +146,22 @@ def test_work_counters_and_timing(rmat14, refs14):
     assert st["kernels_timed"] == st["iterations"] and st["ms_kernel_sum"] > 0
 
 
+def test_static_sweeps_across_many_queries(rmat14):
+    """In-place and flag-scan sweeps skip tiles by improvement stamps kept across queries on
+    one handle: alternate sources, algorithms, K and variants, and check every answer."""
+    n, s, d, w, g = rmat14
+    rng = np.random.default_rng(7)
+    seq = [(alg, int(src), K, var) for alg, src, K, var in
+           zip(rng.choice(list(ALG), 24), rng.integers(0, n, 24), rng.choice([4, 8, 16, 32], 24),
+               rng.choice(["async_push_dd", "sync_push_dd:flags", "sync_push_dd"], 24))]
+    seq[0] = ("bfs", int(np.argmax(np.bincount(s, minlength=n))), 8, "async_push_dd")
+    for alg, src, K, var in seq:
+        variant, _, fr = var.partition(":")
+        lab, _ = g.run(alg, src, variant=variant, frontier=fr or "compact", K=int(K))
+        ref = O.reference_labels(ALG[alg], n, s, d, w, src)
+        assert np.array_equal(lab, ref), (alg, src, K, var)
+
+
 def test_energy_reading_is_opt_in(rmat14, refs14):
     n, s, d, w, g = rmat14
     src, refs = refs14
