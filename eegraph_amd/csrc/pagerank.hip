@@ -271,15 +271,18 @@ __global__ void k_pr_isolated(const uint64_t* __restrict__ in_row, const uint32_
 }
 
 // Sums the per-workgroup residuals in a fixed order and decides convergence.
-__global__ void k_pr_reduce(const double* __restrict__ part, uint32_t nparts, Ctl* ctl, uint32_t it,
-                            double eps) {
-    __shared__ double sRed[WG];
+// One workgroup of RED_WG threads sums the partials in a fixed order (strided per thread, then
+// a tree), so the residual is deterministic.
+constexpr int RED_WG = 1024;
+__global__ __launch_bounds__(RED_WG) void k_pr_reduce(const double* __restrict__ part, uint32_t nparts, Ctl* ctl,
+                                                      uint32_t it, double eps) {
+    __shared__ double sRed[RED_WG];
     if (ctl->done) return;
     double t = 0;
-    for (uint32_t i = threadIdx.x; i < nparts; i += WG) t += part[i];
+    for (uint32_t i = threadIdx.x; i < nparts; i += RED_WG) t += part[i];
     sRed[threadIdx.x] = t;
     __syncthreads();
-    for (int off = WG / 2; off > 0; off >>= 1) {
+    for (int off = RED_WG / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) sRed[threadIdx.x] += sRed[threadIdx.x + off];
         __syncthreads();
     }
@@ -490,7 +493,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         } else if (it == iso_iters + 1) {
             EEG_HIP(hipMemsetAsync(red + grid_pull + grid_fix, 0, grid_iso * sizeof(double), s));
         }
-        k_pr_reduce<<<1, WG, 0, s>>>(red, nparts, ctl, it, eps);
+        k_pr_reduce<<<1, RED_WG, 0, s>>>(red, nparts, ctl, it, eps);
         EEG_HIP(hipGetLastError());
         hipEvent_t ea2;
         EEG_TRY(w.event(2 * (size_t)it, &ea2));
