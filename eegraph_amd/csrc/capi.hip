@@ -21,7 +21,7 @@ eeg_status Work::event(size_t i, hipEvent_t* out) {
 }
 
 Work::~Work() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    for (auto& ge : graphs) (void)hipGraphExecDestroy(ge.exec);
     for (auto e : events) (void)hipEventDestroy(e);
     if (host_ctl) (void)hipHostFree(host_ctl);
 }

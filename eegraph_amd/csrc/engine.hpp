@@ -102,8 +102,15 @@ struct Work {
     DevMem red;         // PR: per-workgroup residual partials
     uint32_t epoch = 1; // iteration stamps already used on stamp/flag
     DevMem devit;       // replayed iteration chunks: {iteration, stamp base}
-    hipGraphExec_t graph_exec = nullptr;  // captured chunk and the arguments it was captured with
-    std::vector<uint8_t> graph_key;
+    // executable graphs of replayed iteration chunks, by the argument bytes they were built
+    // with (most recently used last, at most kGraphCache): queries that alternate algorithms
+    // on one handle reuse their graphs
+    struct GraphEntry {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t kGraphCache = 8;
+    std::vector<GraphEntry> graphs;
     Ctl* host_ctl = nullptr;  // pinned mirrors of Ctl (two, for overlapped polling)
     std::vector<hipEvent_t> events;  // pool: one or two per launch
     eeg_status event(size_t i, hipEvent_t* out);

@@ -589,22 +589,27 @@ eeg_status ensure_tiles(Tigr& t, uint32_t K, hipStream_t s) {
 // every replay starts in the same state pattern.
 constexpr uint32_t GRAPH_CHUNK = 6;
 
-// The executable graph of one chunk for these kernel arguments: built on first use and kept
-// in the work state until the arguments (buffers, grid, kernel) change.  It holds kernel
-// nodes only.
+// The executable graph of one chunk for these kernel arguments (buffers, grid, kernel): built
+// on first use and kept in the work state with the last few others, so queries that
+// alternate algorithms reuse theirs.  It holds kernel nodes only.
 eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga, hipStream_t s,
                        hipGraphExec_t* out) {
     std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid));
     memcpy(key.data(), &ga, sizeof(RelaxArgs));
     memcpy(key.data() + sizeof(RelaxArgs), &kfn, sizeof(kfn));
     memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn), &grid, sizeof(grid));
-    if (w.graph_exec && w.graph_key == key) {
-        *out = w.graph_exec;
-        return EEG_OK;
+    for (size_t i = 0; i < w.graphs.size(); ++i) {
+        if (w.graphs[i].key == key) {
+            Work::GraphEntry hit = std::move(w.graphs[i]);
+            w.graphs.erase(w.graphs.begin() + (long)i);
+            w.graphs.push_back(std::move(hit));
+            *out = w.graphs.back().exec;
+            return EEG_OK;
+        }
     }
-    if (w.graph_exec) {
-        (void)hipGraphExecDestroy(w.graph_exec);
-        w.graph_exec = nullptr;
+    if (w.graphs.size() >= Work::kGraphCache) {  // every earlier run has finished on its stream
+        (void)hipGraphExecDestroy(w.graphs.front().exec);
+        w.graphs.erase(w.graphs.begin());
     }
     // built node by node (no stream capture, which would interact with other threads'
     // HIP calls): relaxation -> tick -> relaxation -> ... in one dependency chain
@@ -641,14 +646,12 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga
         (void)hipGraphDestroy(graph);
         return fail(EEG_ERR_DEVICE, "building the iteration chunk failed: %s", hipGetErrorString(e));
     }
-    e = hipGraphInstantiate(&w.graph_exec, graph, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        w.graph_exec = nullptr;
-        return fail(EEG_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-    }
-    w.graph_key = key;
-    *out = w.graph_exec;
+    if (e != hipSuccess) return fail(EEG_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    w.graphs.push_back(Work::GraphEntry{std::move(key), exec});
+    *out = exec;
     return EEG_OK;
 }
 
