@@ -518,6 +518,16 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             chunk = std::min<uint32_t>(chunk * 2, 16);
             return EEG_OK;
         };
+        // Iterations still needed after the polled one: the residual decays geometrically, so
+        // log(eps / r_k) / log(r_k / r_k-1) (16 when the last two residuals do not show it).
+        auto remaining = [&](const Ctl& h) -> uint32_t {
+            const uint32_t k = h.iters;
+            if (k < 2) return 16;
+            const double r1 = h.residual[k & 3], r0 = h.residual[(k - 1) & 3];
+            if (!(r1 > 0.0) || !(r0 > r1)) return 16;
+            const double need = std::log(eps / r1) / std::log(r1 / r0);
+            return need < 1.0 ? 1u : (uint32_t)std::min(16.0, std::ceil(need));
+        };
         int cur = 0;
         rc = submit(0);
         while (rc == EEG_OK) {
@@ -532,6 +542,10 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
                     rc = fail(EEG_ERR_NOT_CONVERGED, "PageRank residual above %g after %u iterations", eps, max_iters);
                 break;
             }
+            // size the next chunk to the predicted remainder beyond what is in flight (+1), so
+            // few launches run after convergence; a short prediction costs one more poll
+            const uint32_t inflight = (it - 1) - h.iters, rem = remaining(h);
+            chunk = rem > inflight ? std::min<uint32_t>(rem - inflight + 1, 16) : 1u;
             cur ^= 1;
         }
         (void)hipEventDestroy(poll[0]);
