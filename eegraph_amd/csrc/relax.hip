@@ -56,6 +56,7 @@ struct RelaxArgs {
     // the stamp base, read from device memory; it, stampv, the control slots and the queue
     // ping-pong are derived from them, so one captured chunk serves every later chunk and run.
     const uint32_t* dev_it;
+    uint32_t it_off;              // replayed graph: this node's iteration is dev_it[0] + it_off
     uint32_t *qa, *sa, *qb, *sb;  // queue buffers 0 and 1
     // Tile skipping of the static sweeps (M_ASYNC; M_FLAGS except CC, whose flag scan measured
     // slower with it): an improvement of vertex u in iteration it stamps u's 64-vertex block
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
     using R = Rule<ALG>;
     RelaxArgs a = args;
     if (MODE == M_QUEUE && a.dev_it) {
-        const uint32_t it = *a.dev_it;
+        const uint32_t it = *a.dev_it + a.it_off;
         const bool odd = ((it - 1) & 1) != 0;  // input queue = buffer (it-1)&1
         a.it = it;
         a.stampv = a.dev_it[1] + it;
@@ -426,8 +427,8 @@ __global__ void k_mark_blocks(const uint32_t* __restrict__ seeds, uint32_t nseed
 }
 
 // Advances the device iteration counter of a replayed chunk (one thread, vector store).
-__global__ void k_tick(uint32_t* it) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *it += 1;
+__global__ void k_tick(uint32_t* it, uint32_t step) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *it += step;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint64_t n, uint32_t val) {
@@ -612,16 +613,17 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga
         w.graphs.erase(w.graphs.begin());
     }
     // built node by node (no stream capture, which would interact with other threads'
-    // HIP calls): relaxation -> tick -> relaxation -> ... in one dependency chain
+    // HIP calls): GRAPH_CHUNK relaxations in one dependency chain, node i running iteration
+    // dev_it[0] + i, then one tick that advances dev_it[0] by GRAPH_CHUNK
     hipGraph_t graph = nullptr;
     EEG_HIP(hipGraphCreate(&graph, 0));
     RelaxArgs a = ga;
     uint32_t* itp = const_cast<uint32_t*>(ga.dev_it);
     void* rargs[] = {&a};
-    void* targs[] = {&itp};
     hipError_t e = hipSuccess;
     hipGraphNode_t prev = nullptr;
     for (uint32_t i = 0; i < GRAPH_CHUNK && e == hipSuccess; ++i) {
+        a.it_off = i;  // the node keeps its own copy of the arguments
         hipKernelNodeParams kp{};
         kp.func = (void*)kfn;
         kp.gridDim = dim3(grid);
@@ -629,9 +631,13 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga
         kp.sharedMemBytes = 0;
         kp.kernelParams = rargs;
         kp.extra = nullptr;
-        hipGraphNode_t nr = nullptr, nt = nullptr;
+        hipGraphNode_t nr = nullptr;
         e = hipGraphAddKernelNode(&nr, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
-        if (e != hipSuccess) break;
+        prev = nr;
+    }
+    uint32_t step = GRAPH_CHUNK;
+    void* targs[] = {&itp, &step};
+    if (e == hipSuccess) {
         hipKernelNodeParams tp{};
         tp.func = (void*)k_tick;
         tp.gridDim = dim3(1);
@@ -639,8 +645,8 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga
         tp.sharedMemBytes = 0;
         tp.kernelParams = targs;
         tp.extra = nullptr;
-        e = hipGraphAddKernelNode(&nt, graph, &nr, 1, &tp);
-        prev = nt;
+        hipGraphNode_t nt = nullptr;
+        e = hipGraphAddKernelNode(&nt, graph, &prev, 1, &tp);
     }
     if (e != hipSuccess) {
         (void)hipGraphDestroy(graph);
