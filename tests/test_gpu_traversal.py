@@ -108,6 +108,21 @@ def test_grid_with_shortcuts(alg):
             assert np.array_equal(lab, ref), f"{alg} {variant} {frontier}"
 
 
+@pytest.mark.parametrize("alg", list(ALG))
+def test_static_sweeps_on_a_larger_grid(alg):
+    """512 x 512 grid with 1% shortcuts: about a thousand tiles and dozens of sweeps, so the
+    in-place and flag-scan sweeps skip most tiles in most iterations."""
+    s, d, w = eg.gen_grid(512, 512, 0.01, seed=5, wmin=1, wmax=1000)
+    n = 512 * 512
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    src = 12345
+    ref = O.reference_labels(ALG[alg], n, s, d, w, src)
+    for variant, frontier in (("async_push_dd", "compact"), ("sync_push_dd", "flags"), ("sync_push_dd", "compact")):
+        lab, st = g.run(alg, src, variant=variant, K=8, frontier=frontier)
+        assert np.array_equal(lab, ref), f"{alg} {variant} {frontier}"
+    g.close()
+
+
 def test_edge_cases_tiny_graphs():
     """Isolated source, self-loops, duplicates, a single vertex, and brute-force closure."""
     rng = np.random.default_rng(11)
