@@ -459,7 +459,12 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
 
     hipEvent_t ev_start;
     EEG_TRY(w.event(0, &ev_start));
-    auto launch = [&](uint32_t it) -> eeg_status {
+    // a timing event after iteration `it` is recorded only where the run may end (the last
+    // iteration of every chunk and the iterations near the predicted end): each record costs
+    // device time between the launches
+    std::vector<uint8_t> recorded;
+    uint32_t predicted_end = 0xffffffffu;
+    auto launch = [&](uint32_t it, bool rec) -> eeg_status {
         pa.it = it;
         const int ob = inplace ? 0 : (int)((it - 1) & 1), nb = inplace ? 0 : (int)(it & 1);
         pa.x_old = w.fx[ob].as<float>();
@@ -495,9 +500,13 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         }
         k_pr_reduce<<<1, RED_WG, 0, s>>>(red, nparts, ctl, it, eps);
         EEG_HIP(hipGetLastError());
-        hipEvent_t ea2;
-        EEG_TRY(w.event(2 * (size_t)it, &ea2));
-        EEG_HIP(hipEventRecord(ea2, s));
+        if (recorded.size() <= it) recorded.resize((size_t)it + 64, 0);
+        recorded[it] = (rec || o.timing) ? 1 : 0;
+        if (recorded[it]) {
+            hipEvent_t ea2;
+            EEG_TRY(w.event(2 * (size_t)it, &ea2));
+            EEG_HIP(hipEventRecord(ea2, s));
+        }
         (void)ea;
         return EEG_OK;
     };
@@ -512,7 +521,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
         EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
         auto submit = [&](int slot) -> eeg_status {
-            for (uint32_t k = 0; k < chunk; ++k, ++it) EEG_TRY(launch(it));
+            for (uint32_t k = 0; k < chunk; ++k, ++it)
+                EEG_TRY(launch(it, k + 1 == chunk || (predicted_end != 0xffffffffu && it + 2 >= predicted_end)));
             EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
             EEG_HIP(hipEventRecord(poll[slot], s));
             chunk = std::min<uint32_t>(chunk * 2, 16);
@@ -546,6 +556,7 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             // few launches run after convergence; a short prediction costs one more poll
             const uint32_t inflight = (it - 1) - h.iters, rem = remaining(h);
             chunk = rem > inflight ? std::min<uint32_t>(rem - inflight + 1, 16) : 1u;
+            predicted_end = h.iters + rem;
             cur ^= 1;
         }
         (void)hipEventDestroy(poll[0]);
@@ -562,8 +573,13 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     EEG_HIP(hipGetLastError());
     EEG_HIP(hipStreamSynchronize(s));
     if (st) {
+        // the first recorded event at or after the last working iteration (every chunk ends
+        // with one, so at worst a few no-op launches past it)
+        uint32_t ie = std::max<uint32_t>(iters, 1);
+        while (ie < recorded.size() && !recorded[ie]) ++ie;
+        if (ie >= recorded.size()) ie = (uint32_t)recorded.size() - 1;
         hipEvent_t ev_end;
-        EEG_TRY(w.event(2 * (size_t)std::max<uint32_t>(iters, 1), &ev_end));
+        EEG_TRY(w.event(2 * (size_t)ie, &ev_end));
         float ms = 0;
         EEG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end));
         st->iterations = iters;
