@@ -588,9 +588,10 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
         st->pr_residual = hc.residual[iters & 3];
         st->edges_relaxed = (uint64_t)iters * g->m;
         st->vertices_touched = (uint64_t)iters * n;
-        // per iteration: 4 B source index + 4 B contribution gather per in-edge; per vertex
-        // 16 B in-row, 16 B out-row, 4 B old rank, 8 B new rank + contribution
-        st->bytes_alg = (double)iters * (8.0 * (double)g->m + 44.0 * (double)n);
+        // per iteration: 4 B source index + 4 B contribution per in-edge and 8 B per vertex
+        // (rank read and write); the row offsets, out-degrees, second rank word and
+        // contributions this layout also moves (32 B more per vertex) are not counted
+        st->bytes_alg = (double)iters * (8.0 * (double)g->m + 8.0 * (double)n);
         double ksum = 0;
         uint32_t kn = 0;
         if (o.timing) {
