@@ -7,8 +7,10 @@
 //                    gathered per destination over the in-edge CSR with Tigr virtual nodes
 //                    (K in-edges each, K lanes, one edge per lane).  Each virtual node reduces
 //                    its K contributions with a fixed TwoSum tree, each vertex adds its virtual
-//                    nodes in order with compensation (Kahan/TwoSum), vertices that span tiles
-//                    are finished by a fix-up pass in tile order.  Results are deterministic.
+//                    nodes of a tile in order with compensation (TwoSum; runs longer than
+//                    PR_LONG as 64 in-order chunks joined by a fixed tree), vertices that span
+//                    tiles are finished by a fix-up pass in tile order.  Results are
+//                    deterministic.
 //                    The L1 residual is accumulated in FP64 and reduced in a fixed order.
 //   async variants - the same gather with ranks updated in place (Gauss-Seidel in tile order):
 //                    fewer sweeps, no atomics, order of reads not fixed (result within the
@@ -70,9 +72,13 @@ __device__ __forceinline__ void finish_vertex(const PullArgs& a, uint32_t v, Com
 // Edge slots per pull tile: twice the traversal tile, so each lane keeps 16 independent
 // gathers in flight and the per-tile staging is amortised over more edges.
 constexpr int PR_TILE = 2 * TILE_EDGES;
+// Virtual nodes of one entry in one tile beyond which a wave, not a thread, sums them.
+constexpr int PR_LONG = 32;
 
 template <int K>
-__global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
+// At most 64 VGPRs (8 waves per SIMD): the gathers are latency-bound, so the extra resident
+// waves are worth more than the registers (-4% per iteration against 71 VGPRs, 7 waves).
+__global__ __launch_bounds__(WG, 8) void k_pr_pull(PullArgs a) {
     constexpr int VPR = WG / K;
     constexpr int TV = PR_TILE / K;
     constexpr int ROUNDS = PR_TILE / WG;
@@ -176,14 +182,11 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                 if (lj == 0 && t0 + slot < t1) sPart[slot] = make_float2(g.s, g.e);
             }
             __syncthreads();
-            // per entry: add its virtual nodes of this tile in order
-            for (int k = threadIdx.x; k < nent; k += WG) {
-                const uint32_t s0 = max(sS[k], t0), s1 = min(sS[k + 1], t1);
-                Comp acc{0.0f, 0.0f};
-                for (uint32_t t = s0; t < s1; ++t) {
-                    const float2 p = sPart[t - t0];
-                    acc = comp_add(acc, Comp{p.x, p.y});
-                }
+            // Per entry: its virtual nodes of this tile added in order, one thread per entry.  A
+            // run longer than PR_LONG (the hubs: up to a whole tile) is summed by one wave
+            // instead, each lane a contiguous chunk in order and the lanes by a fixed tree, so a
+            // single thread does not serialise the tile.
+            auto finish_entry = [&](int k, Comp acc) {
                 const bool started_before = sS[k] < t0;
                 const bool continues = sS[k + 1] > t1;
                 if (!started_before && !continues) {
@@ -193,6 +196,47 @@ __global__ __launch_bounds__(WG) void k_pr_pull(PullArgs a) {
                     a.spill[2 * tile] = make_float2(acc.s, acc.e);      // head
                 } else {
                     a.spill[2 * tile + 1] = make_float2(acc.s, acc.e);  // tail
+                }
+            };
+            for (int k = threadIdx.x; k < nent; k += WG) {
+                const uint32_t s0 = max(sS[k], t0), s1 = min(sS[k + 1], t1);
+                if (s1 - s0 > (uint32_t)PR_LONG) continue;
+                Comp acc{0.0f, 0.0f};
+                for (uint32_t t = s0; t < s1; ++t) {
+                    const float2 p = sPart[t - t0];
+                    acc = comp_add(acc, Comp{p.x, p.y});
+                }
+                finish_entry(k, acc);
+            }
+            {
+                const int wv = threadIdx.x / WAVE, lane = (int)lane_id();
+                int nlong = 0;  // long entries seen so far; the j-th goes to wave j % waves
+                for (int base = 0; base < nent; base += WAVE) {
+                    const int k = base + lane;
+                    bool lng = false;
+                    if (k < nent) lng = min(sS[k + 1], t1) - max(sS[k], t0) > (uint32_t)PR_LONG;
+                    uint64_t mask = __ballot(lng);
+                    while (mask) {
+                        const int kk = base + (int)(__ffsll((unsigned long long)mask) - 1);
+                        mask &= mask - 1;
+                        if (nlong++ % (WG / WAVE) != wv) continue;
+                        const uint32_t s0 = max(sS[kk], t0), s1 = min(sS[kk + 1], t1);
+                        const uint32_t chunk = (s1 - s0 + WAVE - 1) / WAVE;
+                        const uint32_t q0 = min(s0 + (uint32_t)lane * chunk, s1), q1 = min(q0 + chunk, s1);
+                        Comp acc{0.0f, 0.0f};
+                        for (uint32_t t = q0; t < q1; ++t) {
+                            const float2 p = sPart[t - t0];
+                            acc = comp_add(acc, Comp{p.x, p.y});
+                        }
+#pragma unroll
+                        for (int off = WAVE / 2; off > 0; off >>= 1) {
+                            Comp o;
+                            o.s = __shfl_down(acc.s, off, WAVE);
+                            o.e = __shfl_down(acc.e, off, WAVE);
+                            acc = comp_add(acc, o);
+                        }
+                        if (lane == 0) finish_entry(kk, acc);
+                    }
                 }
             }
             __syncthreads();
