@@ -81,9 +81,11 @@ struct Tigr {
     DevMem tstart;      // first entry (vlist position) of every traversal tile of the static sweep
     DevMem tblk;        // per tile: first and last 64-vertex block of its entries (lo | hi << 32)
     bool tiles_built = false;
-    DevMem span;        // in-edge tables only: positions of vertices crossing a PageRank tile
+    DevMem span;        // in-edge tables only: positions of vertices crossing a PageRank tile:
+                        // nspan crossing few tiles from the front, nspan_long crossing many
+                        // from the back (span_cap entries)
     DevMem prstart;     // in-edge tables only: first entry (vlist position) of each PageRank tile
-    uint32_t nspan = 0;
+    uint32_t nspan = 0, nspan_long = 0, span_cap = 0;
     bool span_built = false;
 };
 

@@ -76,9 +76,10 @@ constexpr int PR_TILE = 2 * TILE_EDGES;
 constexpr int PR_LONG = 32;
 
 template <int K>
-// At most 64 VGPRs (8 waves per SIMD): the gathers are latency-bound, so the extra resident
-// waves are worth more than the registers (-4% per iteration against 71 VGPRs, 7 waves).
-__global__ __launch_bounds__(WG, 8) void k_pr_pull(PullArgs a) {
+// At most 64 VGPRs, 8 waves per SIMD (K = 4: LDS holds 4 workgroups per CU, so 128): the
+// gathers are latency-bound, so the extra resident waves are worth more than the registers
+// (-4% per iteration against 71 VGPRs, 7 waves, at K = 8).
+__global__ __launch_bounds__(WG, K >= 8 ? 8 : 4) void k_pr_pull(PullArgs a) {
     constexpr int VPR = WG / K;
     constexpr int TV = PR_TILE / K;
     constexpr int ROUNDS = PR_TILE / WG;
@@ -255,10 +256,16 @@ __global__ __launch_bounds__(WG, 8) void k_pr_pull(PullArgs a) {
     }
 }
 
+// Tiles crossed by a vertex beyond which a wave, not a thread, combines its partial sums.
+constexpr uint32_t FIX_LONG = 8;
+
 // Vertices whose virtual nodes span tiles: tail of the first tile + heads of the following
-// tiles, in tile order.  `span` lists their vlist positions.
-__global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __restrict__ span,
-                                               uint32_t nspan, uint32_t TV, double* res_part) {
+// tiles, in tile order.  `span` lists their vlist positions: one thread per vertex, and one
+// wave per vertex in `span_long` (the hubs, crossing more than FIX_LONG tiles), whose lanes
+// load the partial sums at once and whose lane 0 adds them in the same order.
+__global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __restrict__ span, uint32_t nspan,
+                                               const uint32_t* __restrict__ span_long, uint32_t nlong,
+                                               uint32_t TV, double* res_part) {
     __shared__ double sRed[WG / WAVE];
     if (a.ctl->done) return;
     double res = 0.0;
@@ -274,6 +281,29 @@ __global__ __launch_bounds__(WG) void k_pr_fix(PullArgs a, const uint32_t* __res
         }
         const uint32_t v = a.vlist[pos];
         finish_vertex(a, v, acc, (double)a.x_old[v] + (double)a.xl_old[v], a.outdeg[v], res);
+    }
+    const int lane = (int)lane_id();
+    const uint32_t nw = gridDim.x * (WG / WAVE);
+    for (uint32_t j = blockIdx.x * (WG / WAVE) + threadIdx.x / WAVE; j < nlong; j += nw) {
+        const uint32_t pos = span_long[j];
+        const uint32_t s0 = a.vscan[pos], s1 = a.vscan[pos + 1];
+        const uint32_t ta = s0 / TV, np = (s1 - 1) / TV - ta + 1;  // parts: tail of ta, heads after
+        // starting from 0 + tail gives the tail's bits exactly, as the thread form does
+        Comp acc{0.0f, 0.0f};
+        for (uint32_t base = 0; base < np; base += WAVE) {
+            const uint32_t p = base + (uint32_t)lane;
+            float2 h = make_float2(0.0f, 0.0f);
+            if (p < np) h = a.spill[p == 0 ? 2 * ta + 1 : 2 * (ta + p)];
+            const uint32_t cnt = min(np - base, (uint32_t)WAVE);
+            for (uint32_t l = 0; l < cnt; ++l) {
+                const float hs = __shfl(h.x, (int)l, WAVE), he = __shfl(h.y, (int)l, WAVE);
+                acc = comp_add(acc, Comp{hs, he});
+            }
+        }
+        if (lane == 0) {
+            const uint32_t v = a.vlist[pos];
+            finish_vertex(a, v, acc, (double)a.x_old[v] + (double)a.xl_old[v], a.outdeg[v], res);
+        }
     }
     res = wave_sum(res);
     if (lane_id() == 0) sRed[threadIdx.x / WAVE] = res;
@@ -377,12 +407,17 @@ __global__ void k_tile_starts(const uint32_t* __restrict__ vscan, uint32_t nz, u
     }
 }
 
-__global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV,
+// Vertices crossing a tile boundary: those crossing at most FIX_LONG tiles from the front of
+// `span`, the others from its back (cap entries: a boundary is crossed by one vertex at most).
+__global__ void k_list_spanning(const uint32_t* __restrict__ vscan, uint32_t nz, uint32_t TV, uint32_t cap,
                                 uint32_t* __restrict__ span, uint32_t* __restrict__ count) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nz;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s0 = vscan[i], s1 = vscan[i + 1];
-        if (s0 / TV != (s1 - 1) / TV) span[atomicAdd(count, 1u)] = (uint32_t)i;
+        const uint32_t ta = s0 / TV, tb = (s1 - 1) / TV;
+        if (ta == tb) continue;
+        if (tb - ta + 1 > FIX_LONG) span[cap - 1 - atomicAdd(&count[1], 1u)] = (uint32_t)i;
+        else span[atomicAdd(&count[0], 1u)] = (uint32_t)i;
     }
 }
 
@@ -406,22 +441,26 @@ eeg_status ensure_pr_span(eeg_graph* g, uint32_t K) {
     const uint64_t ntiles = (t.P + TV - 1) / TV;
     hipStream_t s = g->stream;
     DevMem cnt;
-    EEG_TRY(t.span.alloc((size_t)ntiles * 4 + 4));
-    EEG_TRY(cnt.alloc(4));
-    EEG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    const uint32_t cap = (uint32_t)ntiles + 1;
+    EEG_TRY(t.span.alloc((size_t)cap * 4));
+    EEG_TRY(cnt.alloc(8));
+    EEG_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
     EEG_TRY(t.prstart.alloc((size_t)ntiles * 4 + 4));
     if (t.nz) {
         k_list_spanning<<<std::max(1u, std::min(8192u, (t.nz + WG - 1) / WG)), WG, 0, s>>>(
-            t.vscan.as<uint32_t>(), t.nz, TV, t.span.as<uint32_t>(), cnt.as<uint32_t>());
+            t.vscan.as<uint32_t>(), t.nz, TV, cap, t.span.as<uint32_t>(), cnt.as<uint32_t>());
         EEG_HIP(hipGetLastError());
         k_tile_starts<<<std::max<uint64_t>(1, std::min<uint64_t>(4096, (ntiles + WG - 1) / WG)), WG, 0, s>>>(
             t.vscan.as<uint32_t>(), t.nz, TV, (uint32_t)ntiles, t.prstart.as<uint32_t>());
         EEG_HIP(hipGetLastError());
     }
-    uint32_t nspan = 0;
-    EEG_HIP(hipMemcpyAsync(&nspan, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    uint32_t ns[2] = {0, 0};
+    EEG_HIP(hipMemcpyAsync(ns, cnt.p, 8, hipMemcpyDeviceToHost, s));
     EEG_HIP(hipStreamSynchronize(s));
-    t.nspan = nspan;
+    if ((uint64_t)ns[0] + ns[1] > cap) return fail(EEG_ERR_DEVICE, "PageRank span list overflow");
+    t.nspan = ns[0];
+    t.nspan_long = ns[1];
+    t.span_cap = cap;
     t.span_built = true;
     return EEG_OK;
 }
@@ -460,14 +499,16 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     // vertices whose virtual nodes cross a tile boundary (finished by k_pr_fix)
     EEG_TRY(ensure_pr_span(g, K));
     const DevMem& span = t.span;
-    const uint32_t nspan = t.nspan;
+    const uint32_t nshort = t.nspan, nlong = t.nspan_long;
+    const uint32_t nspan = nshort + nlong;
+    const uint32_t* span_long = span.as<uint32_t>() + (t.span_cap - nlong);
     // the fix-up order only matters within a vertex, so the list order is irrelevant
 
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
     const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : 16;
     const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * wpc));
-    const unsigned grid_fix = std::max(1u, std::min(1024u, (nspan + WG - 1) / WG));
+    const unsigned grid_fix = std::max(1u, std::min(1024u, std::max((nshort + WG - 1) / WG, (nlong + 3) / 4)));
     const unsigned grid_iso = std::max(1u, std::min(4096u, (n + WG - 1) / WG));
     const uint32_t nparts = grid_pull + grid_fix + grid_iso;
     EEG_TRY(w.red.alloc((size_t)nparts * sizeof(double)));
@@ -530,7 +571,8 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             EEG_HIP(hipMemsetAsync(red, 0, grid_pull * sizeof(double), s));
         }
         if (nspan) {
-            k_pr_fix<<<grid_fix, WG, 0, s>>>(pa, span.as<uint32_t>(), nspan, TV, red + grid_pull);
+            k_pr_fix<<<grid_fix, WG, 0, s>>>(pa, span.as<uint32_t>(), nshort, span_long, nlong, TV,
+                                             red + grid_pull);
             EEG_HIP(hipGetLastError());
         }
         const uint32_t iso_iters = inplace ? 1 : 2;  // vertices without in-edges settle at once
