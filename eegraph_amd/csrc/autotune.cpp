@@ -155,7 +155,7 @@ extern "C" eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* 
     // 3. resident workgroups per CU
     {
         const eeg_opts at = win;
-        for (uint32_t w : {8u, 12u, 16u, 24u, 32u}) {
+        for (uint32_t w : {8u, 12u, 16u, 24u, 32u, 48u}) {
             eeg_opts o = at;
             o.wgs_per_cu = w;
             consider(o);

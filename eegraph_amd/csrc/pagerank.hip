@@ -506,7 +506,9 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
 
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
-    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : 16;
+    // workgroups per CU: more than stay resident (8), so tiles are handed out as workgroups
+    // finish; 48 for the Jacobi gather, 24 in place (whose sweep count also depends on it)
+    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : (inplace ? 24u : 48u);
     const unsigned grid_pull = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)cus * wpc));
     const unsigned grid_fix = std::max(1u, std::min(1024u, std::max((nshort + WG - 1) / WG, (nlong + 3) / 4)));
     const unsigned grid_iso = std::max(1u, std::min(4096u, (n + WG - 1) / WG));
