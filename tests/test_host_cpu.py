@@ -161,3 +161,17 @@ def test_entry_points_validate_before_touching_a_device():
     assert L.lib.eeg_run_packed(None, 0, L.BFS, None, None, None, None) == L.ERR_INVALID_ARG
     assert L.lib.eeg_autotune(None, L.BFS, None, 0, None, None) == L.ERR_INVALID_ARG
     assert L.lib.eeg_run_batch(None, L.BFS, None, 0, None, None, None) == L.ERR_INVALID_ARG
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 (or a WORLD_SIZE above 1) exits with status 2 and a
+    message, before anything touches a device."""
+    import subprocess
+    import sys
+    bench = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py")
+    p = subprocess.run([sys.executable, bench, "--gpus", "2"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
+    env = dict(os.environ, WORLD_SIZE="4")
+    p = subprocess.run([sys.executable, bench], capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 2 and "WORLD_SIZE 4" in p.stderr
