@@ -91,7 +91,6 @@ __global__ __launch_bounds__(WG, K >= 8 ? 8 : 4) void k_pr_pull(PullArgs a) {
     __shared__ float2 sPart[TV];
     __shared__ float2 sXo[EMAX];     // old rank (hi, lo) of each entry, loaded with its row
     __shared__ uint32_t sOd[EMAX];   // out-degree of each entry
-    __shared__ uint32_t sMisc[4];
     __shared__ double sRed[WG / WAVE];
 
     if (a.ctl->done) return;
@@ -112,24 +111,16 @@ __global__ __launch_bounds__(WG, K >= 8 ? 8 : 4) void k_pr_pull(PullArgs a) {
         for (uint32_t tile = tb; tile < te; tile += tstep) {
             const uint32_t t0 = tile * TV;
             const uint32_t t1 = min(t0 + (uint32_t)TV, T);
+            // the entry holding vnode t1 (the next tile's first entry; nE past the last tile):
+            // at most TV entries start in (t0, t1], so it lies within the staged window
+            const uint32_t e_next = (tile + 1 < ntiles) ? a.tstart[tile + 1] : nE;
             for (int k = threadIdx.x; k < EMAX; k += WG) {
                 const uint64_t i = (uint64_t)e_start + k;
                 sS[k] = (i <= nE) ? a.vscan[i] : 0xffffffffu;  // vscan[nE] = P closes the last
             }
-            if (threadIdx.x == 0) {
-                sMisc[0] = 1;
-                sMisc[1] = 1;
-            }
             __syncthreads();
-            for (int k = threadIdx.x; k < EMAX; k += WG) {
-                const uint32_t cur = sS[k];
-                const uint32_t nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
-                if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;  // entries in the tile
-                if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;          // ... and up to t1
-            }
-            __syncthreads();
-            const int nent = (int)sMisc[0];
-            const int nnext = (int)sMisc[1];
+            // entries in the tile: up to e_next, and e_next itself when it starts before t1
+            const int nent = (int)(e_next - e_start) + (sS[e_next - e_start] < t1 ? 1 : 0);
             // entry data, including what finishing the vertex needs, so those loads overlap
             // with the row loads instead of trailing the gathers
             for (int k = threadIdx.x; k < nent; k += WG) {
@@ -241,7 +232,7 @@ __global__ __launch_bounds__(WG, K >= 8 ? 8 : 4) void k_pr_pull(PullArgs a) {
                 }
             }
             __syncthreads();
-            if (!a.strided) e_start += (uint32_t)(nnext - 1);
+            if (!a.strided) e_start = e_next;
             else if (tile + tstep < te) e_start = a.tstart[tile + tstep];
         }
     }
