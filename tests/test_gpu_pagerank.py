@@ -67,10 +67,16 @@ def test_pagerank_hub_spans_many_tiles():
     g = eg.Graph.from_edges(s, d, None, n=n)
     row, col, _ = O.csr(n, s, d)
     xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13)
-    for K in (4, 32):
+    for K in (4, 8, 32):
         x, st = g.run("pr", 0, variant="sync_push_dd", K=K)
         assert np.abs(x.astype(np.float64) - xf).sum() < L1_TOL
         assert abs(float(x[0]) - xf[0]) / xf[0] < 1e-6
+        # the same bits on a second run (the hub's many tile partials are combined in a fixed
+        # order by one wave), and the in-place variant within the tolerance
+        x2, _ = g.run("pr", 0, variant="sync_push_dd", K=K)
+        assert np.array_equal(x.view(np.uint32), x2.view(np.uint32))
+        xa, _ = g.run("pr", 0, variant="async_push_dd", K=K)
+        assert np.abs(xa.astype(np.float64) - xf).sum() < L1_TOL
 
 
 def test_pagerank_tiny_graphs():
