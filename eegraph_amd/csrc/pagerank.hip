@@ -343,8 +343,18 @@ __global__ __launch_bounds__(RED_WG) void k_pr_reduce(const double* __restrict__
                                                       uint32_t it, double eps) {
     __shared__ double sRed[RED_WG];
     if (ctl->done) return;
+    // strided per thread, four loads in flight, added in index order (the same sum as one at
+    // a time)
     double t = 0;
-    for (uint32_t i = threadIdx.x; i < nparts; i += RED_WG) t += part[i];
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * RED_WG < nparts; i += 4 * RED_WG) {
+        const double p0 = part[i], p1 = part[i + RED_WG], p2 = part[i + 2 * RED_WG], p3 = part[i + 3 * RED_WG];
+        t += p0;
+        t += p1;
+        t += p2;
+        t += p3;
+    }
+    for (; i < nparts; i += RED_WG) t += part[i];
     sRed[threadIdx.x] = t;
     __syncthreads();
     for (int off = RED_WG / 2; off > 0; off >>= 1) {
