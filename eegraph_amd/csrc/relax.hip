@@ -776,9 +776,9 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     const uint32_t TV = TILE_EDGES / K;
     const uint64_t static_tiles = (t.P + TV - 1) / TV;
     // workgroups per CU of the grid (8 stay resident; more hand out tiles as workgroups finish):
-    // measured on RMAT-22 and the 4096^2 grid, flag scans gain from 48, queues from 24; in-place
-    // sweeps stay at 24 (their sweep counts move either way with the schedule)
-    const uint32_t wpc_static = o.wgs_per_cu ? o.wgs_per_cu : (mode == M_FLAGS ? 48u : 24u);
+    // measured on RMAT-22 and the 4096^2 grid, flag scans gain from 48, in-place sweeps from 40
+    // (32 and 48 each lost on one of the two), queues from 24; topology sweeps stay at 24
+    const uint32_t wpc_static = o.wgs_per_cu ? o.wgs_per_cu : (mode == M_FLAGS ? 48u : mode == M_ASYNC ? 40u : 24u);
     const uint32_t wpc_queue = o.wgs_per_cu ? o.wgs_per_cu : 24;
     const unsigned grid_static =
         (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * wpc_static));

@@ -92,9 +92,9 @@ typedef struct eeg_opts {
     float acc;                 /* unused; kept for the layout (the CLI's --accuracy sets epsilon) */
     double epsilon;            /* PR and PPR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7) */
     uint32_t wgs_per_cu;       /* workgroups per compute unit of the tile kernels' grid
-                                  (0 = default: 24 for frontier queues, in-place and topology
-                                  sweeps, 48 for flag scans and batched passes; PageRank gather
-                                  48, in place 24) */
+                                  (0 = default: 24 for frontier queues and topology sweeps, 40
+                                  for in-place sweeps, 48 for flag scans and batched passes;
+                                  PageRank gather 48, in place 24) */
     uint32_t energy;           /* read the amd-smi energy counter before and after the run into
                                   stats->energy_j (two counter reads, ~0.25 ms each; 0 = off) */
     uint32_t reserved[6];
