@@ -187,3 +187,18 @@ def test_batched_ppr_isolated_and_repeated_sources():
         assert np.abs(out[q].astype(np.float64) - single).sum() < 1e-7, q
     assert abs(out[0][5] - 0.15) < 1e-7  # restart mass stays on a vertex without out-edges
     g.close()
+
+
+def test_pagerank_grid_size_does_not_change_bits(rmat14):
+    """The Jacobi gather's tiles are summed in a fixed order whatever the grid, so the ranks
+    are the same bits for one, 48 (the default) and 255 workgroups per CU, and every grid stays
+    within the tolerance of the FP64 fixpoint in place."""
+    n, s, d, g, row, col = rmat14
+    xf, _, _ = O.pagerank(n, row, col, 0.85, 1e-13)
+    ref, st0 = g.run("pr", 0, variant="sync_push_dd", K=8)
+    for wpc in (1, 48, 255):
+        x, st = g.run("pr", 0, variant="sync_push_dd", K=8, wgs_per_cu=wpc)
+        assert st["iterations"] == st0["iterations"]
+        assert np.array_equal(x.view(np.uint32), ref.view(np.uint32)), f"wgs_per_cu={wpc}"
+        xa, _ = g.run("pr", 0, variant="async_push_dd", K=8, wgs_per_cu=wpc)
+        assert np.abs(xa.astype(np.float64) - xf).sum() < L1_TOL

@@ -228,3 +228,17 @@ def test_replayed_chunks_match_eager_launches(rmat14, alg):
         # iteration counts may differ: labels are updated in place, so a vertex can see a
         # neighbour's improvement from the same iteration
         assert np.array_equal(replayed, O.reference_labels(ref_alg, n, s, d, w, src)), (alg, rep)
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+def test_grid_size_does_not_change_labels(rmat14, refs14, alg):
+    """Labels are the oracle's for every launch geometry: default grids, one workgroup per CU
+    (static tile runs with tails) and 64 per CU (tiles handed out as workgroups finish), in
+    every frontier form."""
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    for variant, frontier in (("sync_push_dd", "compact"), ("sync_push_dd", "flags"),
+                              ("async_push_dd", "compact"), ("async_push_td", "compact")):
+        for wpc in (0, 1, 40, 64):
+            lab, _ = g.run(alg, src, variant=variant, frontier=frontier, K=8, wgs_per_cu=wpc)
+            assert np.array_equal(lab, refs[alg]), f"{alg} {variant} {frontier} wgs_per_cu={wpc}"
