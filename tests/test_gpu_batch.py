@@ -105,3 +105,18 @@ def test_replayed_chunks_match_eager_batches(rmat13, alg, nsrc):
         for q in (0, nsrc - 1):
             single, _ = g.run(alg, int(srcs[q]), variant="sync_push_dd", K=8)
             assert np.array_equal(replayed[q], single), (alg, nsrc, rep, q)
+
+
+@pytest.mark.parametrize("alg", list(ALG))
+@pytest.mark.parametrize("nsrc", [12, 256])
+def test_batch_grid_size_does_not_change_answers(rmat13, alg, nsrc):
+    """The same batched answers for one, 48 (the default) and 64 workgroups per CU, through
+    both plane kernels (12 queries: the per-edge pass; 256: the wavefront-per-row pass)."""
+    n, s, d, w, g = rmat13
+    srcs = np.random.default_rng(7 + nsrc).integers(0, n, nsrc).astype(np.uint32)
+    ref, _ = g.run_batch(alg, srcs, K=8)
+    for wpc in (1, 48, 64):
+        out, _ = g.run_batch(alg, srcs, K=8, wgs_per_cu=wpc)
+        assert np.array_equal(out, ref), (alg, nsrc, wpc)
+    single, _ = g.run(alg, int(srcs[-1]), variant="sync_push_dd", K=8)
+    assert np.array_equal(ref[-1], single)
