@@ -507,7 +507,9 @@ constexpr uint32_t SPARSE_MAX = 8;
 constexpr uint32_t PLANE_DENSE_FROM = 16;
 
 template <int K>
-__global__ __launch_bounds__(WG) void k_plane_push(PlaneArgs args) {
+// At most 168 VGPRs at K <= 8 (3 waves per SIMD instead of 2 at ~193, ~116 B of spill): -10 to -12%
+// per pass for 8 and 16 queries on RMAT-24; 4 waves were no faster than 2.
+__global__ __launch_bounds__(WG, K <= 8 ? 3 : 1) void k_plane_push(PlaneArgs args) {
     PlaneArgs a = args;
     plane_resolve(a);
     constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE, VPR = WG / K;
