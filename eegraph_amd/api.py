@@ -258,7 +258,7 @@ class Server:
 
     def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
                  ppr_epsilon=0.0, discard_labels=False, lane_wgs_per_cu=0, lanes=0,
-                 plane_engines=False, bfs_engine=True):
+                 plane_engines=False, bfs_engine=True, first_iteration=0):
         cfg = L.ServerCfg()
         cfg.label_budget_bytes = label_budget_bytes
         cfg.max_batch = max_batch
@@ -270,6 +270,7 @@ class Server:
                      (0 if bfs_engine else L.SERVER_NO_BFS_ENGINE))
         cfg.lane_wgs_per_cu = lane_wgs_per_cu
         cfg.lanes = lanes
+        cfg.first_iteration = first_iteration
         self.graph = graph  # keeps the graph alive while the server runs
         self.n = graph.info()["n"]
         self.discard = discard_labels

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <string>
 
 #include "engine.hpp"
 
@@ -20,9 +21,18 @@ eeg_status Work::event(size_t i, hipEvent_t* out) {
     return EEG_OK;
 }
 
+eeg_status Work::poll_events() {
+    for (int i = 0; i < 2; ++i)
+        if (!poll[i]) EEG_HIP(hipEventCreateWithFlags(&poll[i], hipEventDisableTiming));
+    return EEG_OK;
+}
+
+// Runs end with a stream synchronisation, so no record of these events is pending here.
 Work::~Work() {
     for (auto& ge : graphs) (void)hipGraphExecDestroy(ge.exec);
     for (auto e : events) (void)hipEventDestroy(e);
+    for (auto e : poll)
+        if (e) (void)hipEventDestroy(e);
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
 
@@ -295,8 +305,11 @@ static eeg_status run_common(eeg_graph* g, uint32_t algo, uint32_t source, const
     const uint32_t isrc = (sourced && g->n) ? g->h_perm[source] : 0;  // internal id
     eeg_status s = (algo == EEG_PR || algo == EEG_PPR) ? run_pagerank(g, algo, isrc, o, st)
                                                         : run_traversal(g, algo, isrc, o, st);
+    // the run's detail message outlives the energy read, which may set its own
+    std::string detail = (s != EEG_OK) ? eeg_last_error() : "";
     if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
-    clear_error();
+    if (s != EEG_OK) fail(s, "%s", detail.c_str());
+    else clear_error();
     if (s != EEG_OK && s != EEG_ERR_NOT_CONVERGED) return s;
     if (out && g->n) {  // back to original vertex order
         const size_t bytes = (size_t)g->n * 4;

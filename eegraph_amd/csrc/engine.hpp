@@ -116,6 +116,11 @@ struct Work {
     Ctl* host_ctl = nullptr;  // pinned mirrors of Ctl (two, for overlapped polling)
     std::vector<hipEvent_t> events;  // pool: one or two per launch
     eeg_status event(size_t i, hipEvent_t* out);
+    // Completion markers of the two in-flight control-block copies.  Created once and kept
+    // for the life of the work state: a run ends with records still pending on its stream,
+    // and an event must not be destroyed while a record of it is pending.
+    hipEvent_t poll[2] = {nullptr, nullptr};
+    eeg_status poll_events();
     ~Work();
 };
 

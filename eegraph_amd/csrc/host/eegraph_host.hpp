@@ -120,6 +120,11 @@ public:
     std::string getFileExtension(std::string fileName);
 };
 
+// src/um_graph.cu / include/um_graph.cuh: the unified-memory twin of Graph, with the same
+// constructor and ReadGraph().  Graphs live in HBM here, so it is the same class and the
+// eegraph_<alg>_um entry points below take it (src/main.cu:45,309-325 compile unchanged).
+using UMGraph = Graph;
+
 // src/virtual_graph.cpp: the Tigr virtual graph.  MakeGraph() uploads the edges and builds the
 // CSR and the virtual-node tables in HBM; the host arrays of the reference layout
 // (nodePointer, edgeList, partNodePointer) are filled only by ExportLayout(), for inspection.
@@ -146,18 +151,18 @@ public:
     eeg_status ExportLayout();
 };
 
-// include/eegraph.cuh:21-39.  The _um forms are kept as names; they run the same HBM-resident
-// path (the unified-memory build is replaced by an explicit device CSR).
+// include/eegraph.cuh:21-39.  The _um forms take a UMGraph (= Graph) and run the same
+// HBM-resident path (the unified-memory build is replaced by an explicit device CSR).
 Result eegraph_bfs(ArgumentParser& arguments, Graph& graph);
 Result eegraph_sssp(ArgumentParser& arguments, Graph& graph);
 Result eegraph_sswp(ArgumentParser& arguments, Graph& graph);
 Result eegraph_cc(ArgumentParser& arguments, Graph& graph);
 Result eegraph_pr(ArgumentParser& arguments, Graph& graph);
-Result eegraph_bfs_um(ArgumentParser& arguments, Graph& graph);
-Result eegraph_sssp_um(ArgumentParser& arguments, Graph& graph);
-Result eegraph_sswp_um(ArgumentParser& arguments, Graph& graph);
-Result eegraph_cc_um(ArgumentParser& arguments, Graph& graph);
-Result eegraph_pr_um(ArgumentParser& arguments, Graph& graph);
+Result eegraph_bfs_um(ArgumentParser& arguments, UMGraph& graph);
+Result eegraph_sssp_um(ArgumentParser& arguments, UMGraph& graph);
+Result eegraph_sswp_um(ArgumentParser& arguments, UMGraph& graph);
+Result eegraph_cc_um(ArgumentParser& arguments, UMGraph& graph);
+Result eegraph_pr_um(ArgumentParser& arguments, UMGraph& graph);
 
 // One driver for all algorithms (what each *_main.cu duplicates in the reference).
 Result eegraph_run(uint32_t algo, ArgumentParser& arguments, Graph& graph);

@@ -606,14 +606,17 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
     eeg_status rc = EEG_OK;
     {
         uint32_t chunk = 4;
-        hipEvent_t poll[2];
-        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
-        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        EEG_TRY(w.poll_events());
+        hipEvent_t* poll = w.poll;
+        bool pending[2] = {false, false};  // a copy of this run is in flight in the slot
+        // queues up to `chunk` iterations, never past max_iters (callers check it <= max_iters)
         auto submit = [&](int slot) -> eeg_status {
-            for (uint32_t k = 0; k < chunk; ++k, ++it)
-                EEG_TRY(launch(it, k + 1 == chunk || (predicted_end != 0xffffffffu && it + 2 >= predicted_end)));
+            const uint32_t cnt = std::min<uint32_t>(chunk, max_iters + 1 - it);
+            for (uint32_t k = 0; k < cnt; ++k, ++it)
+                EEG_TRY(launch(it, k + 1 == cnt || (predicted_end != 0xffffffffu && it + 2 >= predicted_end)));
             EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
             EEG_HIP(hipEventRecord(poll[slot], s));
+            pending[slot] = true;
             chunk = std::min<uint32_t>(chunk * 2, 16);
             return EEG_OK;
         };
@@ -633,12 +636,11 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             if (it <= max_iters) rc = submit(cur ^ 1);
             if (rc != EEG_OK) break;
             EEG_HIP(hipEventSynchronize(poll[cur]));
+            pending[cur] = false;
             const Ctl& h = w.host_ctl[cur];
             if (h.done) break;
-            if (it > max_iters) {
-                EEG_HIP(hipEventSynchronize(poll[cur ^ 1]));
-                if (!w.host_ctl[cur ^ 1].done)
-                    rc = fail(EEG_ERR_NOT_CONVERGED, "PageRank residual above %g after %u iterations", eps, max_iters);
+            if (!pending[cur ^ 1]) {  // max_iters launched and none of them converged
+                rc = fail(EEG_ERR_NOT_CONVERGED, "PageRank residual above %g after %u iterations", eps, max_iters);
                 break;
             }
             // size the next chunk to the predicted remainder beyond what is in flight (+1), so
@@ -648,13 +650,13 @@ eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_
             predicted_end = h.iters + rem;
             cur ^= 1;
         }
-        (void)hipEventDestroy(poll[0]);
-        (void)hipEventDestroy(poll[1]);
     }
     EEG_HIP(hipStreamSynchronize(s));
     EEG_HIP(hipMemcpy(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     const Ctl hc = w.host_ctl[0];
     const uint32_t iters = hc.iters;
+    if (rc == EEG_OK && !hc.done)
+        rc = fail(EEG_ERR_NOT_CONVERGED, "PageRank residual above %g after %u iterations", eps, max_iters);
     // result: the buffer written by the last iteration, copied into the label slot
     EEG_TRY(w.lab.alloc((size_t)n * 4 + 4));
     const int fb = inplace ? 0 : (int)(iters & 1);

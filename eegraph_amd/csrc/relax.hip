@@ -892,9 +892,8 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         ga.sb = w.s[1].as<uint32_t>();
         hipGraphExec_t exec = nullptr;
         EEG_TRY(chunk_graph(w, kfn, grid_queue, ga, s, &exec));
-        hipEvent_t poll[2];
-        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
-        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        EEG_TRY(w.poll_events());
+        hipEvent_t* poll = w.poll;
         uint32_t replays = 0;
         auto submit = [&](int slot) -> eeg_status {
             EEG_HIP(hipGraphLaunch(exec, s));
@@ -921,16 +920,13 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
             }
             cur ^= 1;
         }
-        (void)hipEventDestroy(poll[0]);
-        (void)hipEventDestroy(poll[1]);
         graph_launches = 1 + replays * GRAPH_CHUNK;
     } else {
         // Chunks of launches; the next chunk is queued before the previous chunk's control
         // block is inspected, so the GPU never waits for the host between chunks.
         uint32_t chunk = 2;
-        hipEvent_t poll[2];
-        EEG_HIP(hipEventCreateWithFlags(&poll[0], hipEventDisableTiming));
-        EEG_HIP(hipEventCreateWithFlags(&poll[1], hipEventDisableTiming));
+        EEG_TRY(w.poll_events());
+        hipEvent_t* poll = w.poll;
         auto submit = [&](int slot) -> eeg_status {
             for (uint32_t k = 0; k < chunk; ++k, ++it) EEG_TRY(launch(it));
             EEG_HIP(hipMemcpyAsync(&w.host_ctl[slot], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -952,8 +948,6 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
             }
             cur ^= 1;
         }
-        (void)hipEventDestroy(poll[0]);
-        (void)hipEventDestroy(poll[1]);
     }
     EEG_HIP(hipStreamSynchronize(s));
     EEG_HIP(hipMemcpy(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost));

@@ -56,7 +56,12 @@ enum EngKind { E_BFS = 0, E_SSSP = 1, E_SSWP = 2 };
 struct EvHold {  // one event shared by the answers gathered in one round
     hipEvent_t e = nullptr;
     ~EvHold() {
-        if (e) (void)hipEventDestroy(e);
+        // the last holder may be a query that was never waited for: its record may still be
+        // pending on the engine stream, and an event is not destroyed while one is
+        if (e) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
     }
 };
 
@@ -118,10 +123,16 @@ struct Engine {
     uint32_t it = 1;                  // next iteration (BFS: level)
     uint32_t base = 1;                // stamp base
     unsigned grid = 0;                // resident workgroups of the iteration kernels
+    // The iteration counter only grows while queries run: BFS levels are stored as
+    // it - 1 + depth and stamps as base + it.  From kItLimit on, no query is admitted until
+    // the engine drains; then every column is reset and the counter restarts at 1, long before
+    // a level or stamp could reach 2^32.
+    static constexpr uint32_t kItLimit = 0xf0000000u;
 
     uint32_t fill() const { return kind == E_SSWP ? 0u : INF; }
     uint32_t seed_val() const { return kind == E_SSWP ? INF : 0u; }
     eeg_status init(eeg_graph* g);
+    eeg_status reset_device();
     void loop();
     eeg_status launch_iteration(eeg_graph* g, uint32_t j);
     bool gather(std::vector<std::pair<TicketP, uint32_t>>& fin);
@@ -219,6 +230,17 @@ eeg_status Engine::init(eeg_graph* g) {
     EEG_TRY(stage.alloc(4 * stage_words * 4));
     EEG_HIP(hipHostMalloc((void**)&hctl, 4 * sizeof(EngCtl), hipHostMallocDefault));
     EEG_HIP(hipHostMalloc((void**)&hstage, 4 * stage_words * 4, hipHostMallocDefault));
+    EEG_TRY(reset_device());
+    EEG_HIP(hipStreamSynchronize(st));
+    col.assign(ncols, nullptr);
+    released.assign(ncols, 0);
+    return EEG_OK;
+}
+
+// Every column to its initial labels, bits and stamps cleared, control block drained (on the
+// engine stream, after everything queued before).
+eeg_status Engine::reset_device() {
+    const uint64_t nb = (uint64_t)n * B;
     k_eng_fill32<<<grid_sv(nb), WG, 0, st>>>(planes.as<uint32_t>(), nb, fill());
     EEG_HIP(hipGetLastError());
     EEG_HIP(hipMemsetAsync(m1.p, 0, m1.bytes, st));
@@ -228,9 +250,6 @@ eeg_status Engine::init(eeg_graph* g) {
     // a drained engine: the first admission starts from empty frontier slots
     k_eng_fill32<<<1, 64, 0, st>>>(&ctlbuf.as<EngCtl>()->ctl.done, 1, 1u);
     EEG_HIP(hipGetLastError());
-    EEG_HIP(hipStreamSynchronize(st));
-    col.assign(ncols, nullptr);
-    released.assign(ncols, 0);
     return EEG_OK;
 }
 
@@ -319,7 +338,7 @@ void Engine::loop() {
     while (true) {
         std::vector<uint32_t> pv, pc;
         uint64_t clr[BATCH_MAXW] = {0, 0, 0, 0};
-        bool any_clr = false;
+        bool any_clr = false, restart = false;
         uint32_t active = 0;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
@@ -329,6 +348,16 @@ void Engine::loop() {
                    (pending.empty() || count_free() == 0))
                 cv.wait(lk);
             if (srv->stop) break;
+            if (it >= kItLimit && running == 0 && inflight.empty()) {
+                // drained past the counter limit: every column is free or given back (its
+                // answer already gathered), so all of them are reset and the counter restarts
+                for (uint32_t c = 0; c < ncols; ++c) {
+                    released[c] = 0;
+                    col[c] = nullptr;
+                }
+                it = 1;
+                restart = true;
+            }
             // Columns given back are reset (one pass over the plane for all of them, in stream
             // order before any reuse) only when waiting queries need them or the engine is idle.
             const size_t nfree = count_free(), nrel = count_released();
@@ -343,7 +372,7 @@ void Engine::loop() {
             }
             uint32_t c = 0;
             const double tnow = srv->now_ms();
-            while (!pending.empty() && !broken) {
+            while (!pending.empty() && !broken && it < kItLimit) {
                 while (c < ncols && col[c]) ++c;
                 if (c >= ncols) break;
                 TicketP tk = pending.front();
@@ -369,7 +398,8 @@ void Engine::loop() {
                 if (col[k] && col[k]->state == T_RUNNING) col[k]->batch = std::max(col[k]->batch, active);
         }
         eeg_status rc = EEG_OK;
-        if (any_clr) {
+        if (restart) rc = reset_device();
+        if (rc == EEG_OK && any_clr) {
             // a converged plane column has no change bits left; a BFS column keeps its seen
             // bits and visit bits on vertices without out-edges
             rc = launch_release(n, W, R, B, clr, planes.as<uint32_t>(), fill(),
@@ -652,6 +682,7 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         e.kind = k;
         e.K = K;
         e.ncols = ncols;
+        if (c.first_iteration) e.it = c.first_iteration;
         EEG_TRY(e.init(g));
     }
     EEG_HIP(hipStreamCreateWithFlags(&srv->io_st, hipStreamNonBlocking));

@@ -269,7 +269,10 @@ typedef struct eeg_server_cfg {
     uint32_t lane_wgs_per_cu;    /* resident workgroups per CU of a single-query lane (0 = 16),
                                     so its kernels leave room for other lanes and the engines */
     uint32_t lanes;              /* single-query lanes (0 = 2, <= 64) */
-    uint32_t reserved[5];
+    uint32_t first_iteration;    /* testing: the engines' iteration counter at start (0 = 1);
+                                    from 0xf0000000 on an engine admits nothing until it drains,
+                                    then resets its columns and restarts the counter at 1 */
+    uint32_t reserved[4];
 } eeg_server_cfg;
 /* Answers are not kept: columns are given back as soon as a query converges and wait() reports
  * status and times only (labels_out must be NULL).  For throughput runs. */

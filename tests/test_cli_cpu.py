@@ -48,3 +48,39 @@ def test_unreadable_graph_fails_with_status():
 def test_reading_reports_reference_counts():
     rc, out = run("cc", "--input", os.path.join(GOLD, "loader_zero.el"))
     assert "Number of nodes = 4" in out and "Number of edges = 4" in out
+
+
+UM_CALLER = r"""
+// The unified-memory callers of src/main.cu:45,309-325: a UMGraph read from a file and handed
+// to the eegraph_<alg>_um entry points.
+#include "eegraph_host.hpp"
+
+int main(int argc, char** argv) {
+    ArgumentParser arguments(argc, argv, true, false);
+    Result result;
+    UMGraph um_graph(arguments.input, true);
+    um_graph.ReadGraph();
+    result = eegraph_bfs_um(arguments, um_graph);
+    result = eegraph_cc_um(arguments, um_graph);
+    result = eegraph_pr_um(arguments, um_graph);
+    result = eegraph_sssp_um(arguments, um_graph);
+    result = eegraph_sswp_um(arguments, um_graph);
+    return (int)result.time;
+}
+"""
+
+
+def test_unified_memory_callers_compile_against_the_host_surface(tmp_path):
+    """The reference's _um call sites build unchanged against host/eegraph_host.hpp and link
+    with the host objects and libeegraph.so."""
+    src = tmp_path / "um_caller.cpp"
+    src.write_text(UM_CALLER)
+    obj = os.path.join(ROOT, "build", "obj")
+    host_objs = [os.path.join(obj, "host_argument_parsing.o"), os.path.join(obj, "host_eegraph.o")]
+    if not all(os.path.exists(o) for o in host_objs):
+        pytest.skip("host objects not built (make -C eegraph_amd/csrc)")
+    exe = tmp_path / "um_caller"
+    p = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "eegraph_amd", "csrc", "host"), str(src)]
+                       + host_objs + ["-L", os.path.join(ROOT, "eegraph_amd", "lib"), "-leegraph", "-lpthread",
+                                      "-o", str(exe)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
