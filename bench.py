@@ -44,8 +44,9 @@ def parse():
     p.add_argument("--K", type=int, default=8)
     p.add_argument("--variants", default="sync,async")
     p.add_argument("--algs", default=",".join(ALGS))
-    p.add_argument("--energy-seconds", type=float, default=1.0,
-                   help="minimum loop length per job for the energy reading")
+    p.add_argument("--energy-seconds", type=float, default=3.5,
+                   help="loop length per job for the energy reading (the counter steps about once "
+                        "per second, so at least 3 s)")
     p.add_argument("--cpu-cap", type=float, default=12.0,
                    help="seconds per algorithm for the CPU baseline (extrapolated past it)")
     p.add_argument("--no-cpu", action="store_true")
@@ -53,10 +54,12 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched and packed section")
     p.add_argument("--no-serve", action="store_true", help="skip the serving section")
     p.add_argument("--no-grid", action="store_true", help="skip the grid section")
+    p.add_argument("--no-sweep", action="store_true", help="skip the K x variant sweep and the autotuner")
     p.add_argument("--batch-scale", type=int, default=24)
     p.add_argument("--serve-scale", type=int, default=26)
-    p.add_argument("--serve-qps", type=float, default=100.0)
-    p.add_argument("--serve-seconds", type=float, default=3.0)
+    p.add_argument("--serve-qps", default="25,50,100,200,400,1000,5000",
+                   help="offered loads (queries/s) of the serving sweep")
+    p.add_argument("--serve-queries", type=int, default=200, help="queries per serving point")
     p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp"),
                    help="only run two multi-source passes (the second timed) of this algorithm on the "
                         "batched section's graph and sources, for profiling the batched kernels")
@@ -140,8 +143,144 @@ def cpu_baseline(scale, ef, seed, K, algs, cap):
                        f"({', '.join(a for a in algs if a != 'pr')}) and serial delta-push PageRank, "
                        f"one query each, 1 thread, cap {cap:.0f} s per algorithm"
                        + (" (capped runs: lower bound on time)" if extrapolated else "")),
-            "per_algorithm": parts, "host_cpu": platform.processor() or platform.machine(),
-            "nproc": os.cpu_count()}
+            "per_algorithm": parts, "host_cpu": cpu_model(), "nproc": os.cpu_count()}
+
+
+class EnergyMeter:
+    """Samples the amd-smi energy accumulator in a thread.  The counter advances in steps (about
+    once per second on MI355X), so power over a region is taken between the first and the last
+    counter step inside it, from the counter's own values and timestamps."""
+
+    def __init__(self, eg):
+        import threading
+
+        self.eg = eg
+        self.samples = []  # (wall s, microjoules, counter ns) at every counter step
+        self.stop = False
+        first = eg.energy_read_ts(0)
+        self.ok = first is not None
+        if self.ok:
+            self.samples.append((time.perf_counter(), first[0], first[1]))
+            self.th = threading.Thread(target=self._loop, daemon=True)
+            self.th.start()
+
+    def _loop(self):
+        while not self.stop:
+            r = self.eg.energy_read_ts(0)
+            if r is not None and r[1] != self.samples[-1][2]:
+                self.samples.append((time.perf_counter(), r[0], r[1]))
+            time.sleep(0.002)
+
+    def close(self):
+        self.stop = True
+        if self.ok:
+            self.th.join()
+
+    def power(self, t0, t1):
+        """Average watts between the first and last counter steps observed in [t0, t1] (wall
+        clock), with the seconds covered; None with fewer than two steps."""
+        inside = [x for x in self.samples if t0 <= x[0] <= t1]
+        if len(inside) < 2:
+            return None
+        a, b = inside[0], inside[-1]
+        dt = (b[2] - a[2]) * 1e-9
+        return ((b[1] - a[1]) * 1e-6 / dt, dt) if dt > 0 else None
+
+    def period_ms(self):
+        import numpy as np
+
+        ts = [x[2] for x in self.samples]
+        return float(np.median(np.diff(ts))) * 1e-6 if len(ts) > 2 else None
+
+
+def energy_section(eg, run, jobs, seconds):
+    """Idle power after a settle period, then each job looped for `seconds`: average watts
+    between counter steps inside the loop, J/run = watts x seconds per run, and the same above
+    idle (idle watts subtracted)."""
+    meter = EnergyMeter(eg)
+    if not meter.ok:
+        return None
+    time.sleep(2.0)  # settle: the previous GPU work is past the counter's next step
+    t = time.perf_counter()
+    time.sleep(max(3.5, seconds))
+    idle = meter.power(t, time.perf_counter())
+    out = {"counter_period_ms": None, "idle_w": round(idle[0], 1) if idle else None,
+           "method": "counter steps inside each loop (the accumulator advances about once per second)"}
+    for alg, var in jobs:
+        runs, t = 0, time.perf_counter()
+        while time.perf_counter() - t < seconds or runs < 3:
+            run(alg, var)
+            runs += 1
+        t1 = time.perf_counter()
+        p = meter.power(t, t1)
+        per_run = (t1 - t) / runs
+        e = {"runs": runs, "s_per_run": round(per_run, 6)}
+        if p:
+            e.update({"avg_w": round(p[0], 1), "j_per_run": round(p[0] * per_run, 5), "window_s": round(p[1], 2)})
+            if idle:
+                e["j_per_run_above_idle"] = round((p[0] - idle[0]) * per_run, 5)
+        out[f"{alg}/{var}"] = e
+    out["counter_period_ms"] = meter.period_ms()
+    meter.close()
+    return out
+
+
+def cpu_model():
+    """The host CPU's model string (first "model name" of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def sweep_section(g, a, src, m):
+    """The RMAT-22 workload over K in {4, 8, 16, 32} and all four variants (BFS/SSSP/SSWP/CC),
+    PageRank sync and async per K: device traversal ms (median of 3 after one untimed run)."""
+    from eegraph_amd import _lib as L
+
+    out = {}
+    for K in (4, 8, 16, 32):
+        g.prepare(K, L.PREP_TIGR | L.PREP_SYM | L.PREP_TRANSPOSE)
+        row = {}
+        for alg in ("bfs", "sssp", "sswp", "cc", "pr"):
+            variants = ("sync_push_dd", "async_push_dd") if alg == "pr" else \
+                ("async_push_td", "async_push_dd", "sync_push_td", "sync_push_dd")
+            r = {}
+            for var in variants:
+                g.run(alg, src, variant=var, K=K, want_labels=False)
+                ms = sorted(g.run(alg, src, variant=var, K=K, want_labels=False)[1]["ms_traversal"]
+                            for _ in range(3))[1]
+                r[var] = {"ms": round(ms, 4), "gteps": round(m / ms / 1e6, 2)}
+            row[alg] = r
+        out[f"K{K}"] = row
+        log(f"sweep K={K}: " + ", ".join(f"{alg} {min(v['ms'] for v in r.values()):.3f} ms" for alg, r in row.items()))
+    return out
+
+
+def autotune_section(g, src, default_K):
+    """eeg_autotune per algorithm against the default options (sync_push_dd, compact queues,
+    K = 8, default grid): median device traversal ms of 5 runs each, same source."""
+    import eegraph_amd as eg
+
+    res = {}
+    for alg in ("bfs", "sssp", "sswp", "cc", "pr"):
+        o, tuned_ms = g.autotune(alg, [src])
+        d = eg.make_opts(variant="sync_push_dd", K=default_K)
+
+        def med(opts):
+            g.run(alg, src, opts=opts, want_labels=False)
+            return sorted(g.run(alg, src, opts=opts, want_labels=False)[1]["ms_traversal"] for _ in range(5))[2]
+
+        dm, tm = med(d), med(o)
+        res[alg] = {"default_ms": round(dm, 4), "tuned_ms": round(tm, 4), "speedup": round(dm / tm, 3),
+                    "tuned": {"variant": int(o.variant), "K": int(o.K), "block": int(o.block or 256),
+                              "frontier": int(o.frontier), "wgs_per_cu": int(o.wgs_per_cu)}}
+        log(f"autotune {alg}: {res[alg]}")
+    return res
 
 
 def pct(xs, q):
@@ -162,6 +301,7 @@ def batched_section(eg, a):
     g = eg.Graph.rmat(a.batch_scale, 16, seed=4)
     n = g.info()["n"]
     srcs = np.random.default_rng(4).integers(0, n, 256).astype(np.uint32)
+    res[f"rmat{a.batch_scale}_single_query"] = single_query_section(g, a.K)
     multi = {}
     for alg in ("bfs", "sssp", "sswp"):
         g.run(alg, int(srcs[0]), want_labels=False)
@@ -209,6 +349,31 @@ def batched_section(eg, a):
     return res
 
 
+def single_query_section(g, K):
+    """One query per algorithm and variant (sync_push_dd, async_push_dd) from the maximum
+    out-degree vertex of the batched section's graph: device traversal ms (median of 3 after
+    one untimed run), GTEPS on the input edges, and the achieved fraction of the 8 TB/s HBM peak
+    on the algorithmic bytes counted by a separate run (count_work)."""
+    from eegraph_amd import _lib as L
+
+    info = g.info()
+    src, m = info["argmax_out_degree"], info["m"]
+    g.prepare(K, L.PREP_TIGR | L.PREP_SYM | L.PREP_TRANSPOSE)
+    out = {"source": src, "edges": m}
+    for alg in ("bfs", "sssp", "sswp", "cc", "pr"):
+        for var in ("sync_push_dd", "async_push_dd"):
+            c = g.run(alg, src, variant=var, K=K, want_labels=False, count_work=True)[1]
+            g.run(alg, src, variant=var, K=K, want_labels=False)
+            ms = sorted(g.run(alg, src, variant=var, K=K, want_labels=False)[1]["ms_traversal"]
+                        for _ in range(3))[1]
+            gbs = c["bytes_alg"] / (ms * 1e-3) / 1e9
+            out[f"{alg}/{var}"] = {"ms": round(ms, 4), "gteps": round(m / ms / 1e6, 2),
+                                   "iterations": c["iterations"], "alg_bytes": c["bytes_alg"],
+                                   "alg_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            log(f"single query rmat24 {alg}/{var}: {out[f'{alg}/{var}']}")
+    return out
+
+
 def batched_only(eg, a):
     """Two multi-source passes of one algorithm (RMAT scale --batch-scale, EF 16, seed 4, the
     same 256 sources as the batched section); prints the timed pass as one JSON line."""
@@ -254,12 +419,13 @@ def grid_section(eg, a):
 
 
 def serving_section(eg, a):
-    """Serving loop on RMAT scale 26 (EF 16, seed 6, w ~ U{1..64}): Poisson arrivals at
-    --serve-qps for --serve-seconds, mix 40% BFS / 30% SSSP / 20% SSWP / 10% PPR (damping 0.85,
-    one-hot restart), sources ~ U[0, N).  Server: BFS engine (continuous batching), other
-    queries on 2 concurrent single-query lanes, answers not kept.  Baseline: the same queries
-    run one at a time (eeg_run each, serially); its latencies are those of a FIFO queue with
-    the measured service times on the same arrival times.  Latency = answer time - arrival."""
+    """Serving loop on RMAT scale 26 (EF 16, seed 6, w ~ U{1..64}), offered loads swept over
+    --serve-qps (Poisson arrivals, seed 6), each point a stream of --serve-queries queries mixed
+    40% BFS / 30% SSSP / 20% SSWP / 10% PPR (damping 0.85, one-hot restart), sources ~ U[0, N).
+    Server: default configuration (BFS and PPR engines with continuous batching, SSSP/SSWP on
+    two single-query lanes), answers not kept.  Baseline: the same queries run one at a time
+    (eeg_run each, serially); its latencies are those of a FIFO queue with the measured service
+    times on the same arrival times.  Latency = answer time - arrival."""
     import threading
     import numpy as np
 
@@ -269,70 +435,85 @@ def serving_section(eg, a):
     g.prepare(8, 1 | 4)  # Tigr of the out- and in-edge CSRs (PPR gathers over in-edges)
     log(f"serving graph n={n} m={g.info()['m']} built in {time.perf_counter() - t:.2f} s")
     rng = np.random.default_rng(6)
-    nq = max(1, int(a.serve_qps * a.serve_seconds))
+    nq = a.serve_queries
     mix = rng.choice(["bfs", "sssp", "sswp", "ppr"], size=nq, p=[0.4, 0.3, 0.2, 0.1])
     srcs = rng.integers(0, n, nq)
-    gaps = rng.exponential(1.0 / a.serve_qps, nq)
-    arrive = np.cumsum(gaps)
-    srv = eg.Server(g, discard_labels=True)
-    for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
-        srv.wait(srv.submit(alg, 0))
-    tickets = [None] * nq
-    e0 = eg.energy_read(0)
+    gaps = rng.exponential(1.0, nq)  # unit-rate gaps, scaled per offered load
 
-    def producer():
-        t0 = time.perf_counter()
-        for i in range(nq):
-            d = arrive[i] - (time.perf_counter() - t0)
-            if d > 0:
-                time.sleep(d)
-            tickets[i] = srv.submit(mix[i], int(srcs[i]))
-
-    th = threading.Thread(target=producer)
-    th.start()
-    lat, per, first, last, tprint = [], {}, None, 0.0, time.time()
-    for i in range(nq):
-        while tickets[i] is None:
-            time.sleep(0.0002)
-        _, r = srv.wait(tickets[i])
-        lat.append(r["t_done_ms"] - r["t_submit_ms"])
-        per.setdefault(mix[i], []).append(lat[-1])
-        first = r["t_submit_ms"] if first is None else min(first, r["t_submit_ms"])
-        last = max(last, r["t_done_ms"])
-        if time.time() - tprint > 5:
-            tprint = time.time()
-            log(f"serving: {i + 1}/{nq} answered")
-    th.join()
-    e1 = eg.energy_read(0)
-    served_s = (last - first) / 1e3
-    srv.close()
-    log("served stream done; one-at-a-time run of the same queries ...")
+    # one at a time: service time of every query (eeg_run each, serially)
     o = eg.make_opts(variant="sync_push_dd", K=8)
+    for alg in ("bfs", "sssp", "sswp", "ppr"):
+        g.run(alg, 0, opts=o, want_labels=False)
+    meter = EnergyMeter(eg)
     svc = []
-    e2 = eg.energy_read(0)
+    t1 = time.perf_counter()
     for i in range(nq):
         t = time.perf_counter()
         g.run(mix[i], int(srcs[i]), opts=o, want_labels=False)
         svc.append(time.perf_counter() - t)
-    e3 = eg.energy_read(0)
-    fin, lat1 = 0.0, []
-    for i in range(nq):
-        fin = max(arrive[i], fin) + svc[i]
-        lat1.append((fin - arrive[i]) * 1e3)
-    one_s = fin - arrive[0] + svc[0]
+    t2 = time.perf_counter()
+    p1 = meter.power(t1, t2)
+    one_j = round(p1[0] * (t2 - t1) / nq, 4) if p1 else None
+    log(f"one at a time: {nq} queries in {t2 - t1:.2f} s ({nq / (t2 - t1):.1f} q/s)")
+
+    srv = eg.Server(g, discard_labels=True)
+    for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
+        srv.wait(srv.submit(alg, 0))
+    points = []
+    for rate in [float(x) for x in a.serve_qps.split(",")]:
+        arrive = np.cumsum(gaps) / rate
+        tickets = [None] * nq
+
+        def producer():
+            t0 = time.perf_counter()
+            for i in range(nq):
+                d = arrive[i] - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(mix[i], int(srcs[i]))
+
+        th = threading.Thread(target=producer)
+        tw0 = time.perf_counter()
+        th.start()
+        lat, per, first, last, tprint = [], {}, None, 0.0, time.time()
+        for i in range(nq):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            _, r = srv.wait(tickets[i])
+            lat.append(r["t_done_ms"] - r["t_submit_ms"])
+            per.setdefault(str(mix[i]), []).append(lat[-1])
+            first = r["t_submit_ms"] if first is None else min(first, r["t_submit_ms"])
+            last = max(last, r["t_done_ms"])
+            if time.time() - tprint > 5:
+                tprint = time.time()
+                log(f"serving {rate:g} q/s: {i + 1}/{nq} answered")
+        th.join()
+        tw1 = time.perf_counter()
+        ps = meter.power(tw0, tw1)
+        served_s = (last - first) / 1e3
+        fin, lat1 = 0.0, []
+        for i in range(nq):
+            fin = max(arrive[i], fin) + svc[i]
+            lat1.append((fin - arrive[i]) * 1e3)
+        one_s = fin - arrive[0]
+        pt = {"offered_qps": rate,
+              "served": {"qps": round(nq / served_s, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                         "per_algorithm_p50_p99_ms": {k: [pct(v, 50), pct(v, 99)] for k, v in per.items()},
+                         "j_per_query": round(ps[0] * (tw1 - tw0) / nq, 4) if ps else None},
+              "one_at_a_time": {"qps": round(nq / one_s, 1), "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99)}}
+        pt["qps_ratio"] = round(pt["served"]["qps"] / pt["one_at_a_time"]["qps"], 2)
+        points.append(pt)
+        log(f"serving point: {pt}")
+    srv.close()
+    meter.close()
     g.close()
-    out = {"workload": f"rmat{a.serve_scale}_ef16_seed6, Poisson {a.serve_qps:g} q/s x {a.serve_seconds:g} s, "
+    out = {"workload": f"rmat{a.serve_scale}_ef16_seed6, {nq} queries per point, Poisson arrivals, "
                        f"40/30/20/10 bfs/sssp/sswp/ppr",
            "queries": nq,
-           "served": {"qps": round(nq / served_s, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                      "per_algorithm_p50_p99_ms": {str(k): [pct(v, 50), pct(v, 99)] for k, v in per.items()},
-                      "j_per_query": round((e1 - e0) * 1e-6 / nq, 4) if e0 is not None else None},
-           "one_at_a_time": {"qps": round(nq / one_s, 1), "capacity_qps": round(nq / sum(svc), 1),
-                             "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99),
-                             "j_per_query": round((e3 - e2) * 1e-6 / nq, 4) if e2 is not None else None}}
-    out["one_at_a_time"]["qps"] = float(out["one_at_a_time"]["qps"])
-    out["qps_ratio"] = round(float(out["served"]["qps"] / out["one_at_a_time"]["qps"]), 2)
-    log(f"serving: {out}")
+           "one_at_a_time_capacity_qps": round(nq / sum(svc), 1),
+           "one_at_a_time_j_per_query": one_j,
+           "points": points,
+           "qps_ratio": points[-1]["qps_ratio"] if points else None}
     return out
 
 
@@ -427,24 +608,12 @@ def main():
                     "alg_bytes_per_launch": round(bpl), "avg_launch_ms": round(avg_ms, 5)}
 
     energy = None
-    if not a.no_energy and eg.energy_read(0) is not None:
-        energy = {}
-        e0 = eg.energy_read(0)
-        time.sleep(1.0)
-        idle_w = (eg.energy_read(0) - e0) * 1e-6 / 1.0
-        energy["idle_w"] = round(idle_w, 1)
-        for alg, var in jobs:
-            runs, t = 0, time.perf_counter()
-            e0 = eg.energy_read(0)
-            while time.perf_counter() - t < a.energy_seconds or runs < 3:
-                run(alg, var)
-                runs += 1
-            dt = time.perf_counter() - t
-            dj = (eg.energy_read(0) - e0) * 1e-6
-            energy[f"{alg}/{var}"] = {"j_per_run": round(dj / runs, 5), "avg_w": round(dj / dt, 1),
-                                      "runs": runs}
-        log("energy done")
+    if not a.no_energy:
+        energy = energy_section(eg, run, jobs, a.energy_seconds)
+        log(f"energy done: idle {energy and energy.get('idle_w')} W")
 
+    sweep = None if a.no_sweep else sweep_section(g, a, src, m)
+    tuned = None if a.no_sweep else autotune_section(g, src, a.K)
     cpu = None
     if not a.no_cpu:
         log("cpu baseline ...")
@@ -471,9 +640,15 @@ def main():
                                f"source=argmax out-degree ({src})",
                    "scale": a.scale, "edge_factor": a.edge_factor, "seed": a.seed, "K": a.K,
                    "edges": m, "queries_per_step": len(jobs)},
+        "timing": "value: wall time of the step's queries with the graph resident in HBM, labels "
+                  "left on the device (no copy-out or un-permute, which add n x 4 bytes over PCIe "
+                  "per query); per-algorithm ms: device events from the first kernel to the end "
+                  "of the last working iteration",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "per_algorithm": per_alg,
+        "sweep_rmat22": sweep,
+        "autotune_rmat22": tuned,
         "energy": energy,
         "grid": grid,
         "batched": batched,

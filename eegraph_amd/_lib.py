@@ -91,7 +91,7 @@ class ServerCfg(C.Structure):
                 ("reserved", u32 * 4)]
 
 
-SERVER_DISCARD_LABELS, SERVER_PLANE_ENGINES, SERVER_NO_BFS_ENGINE = 1, 2, 4
+SERVER_DISCARD_LABELS, SERVER_PLANE_ENGINES, SERVER_NO_BFS_ENGINE, SERVER_NO_PPR_ENGINE = 1, 2, 4, 8
 
 
 class QueryResult(C.Structure):

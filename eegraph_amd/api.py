@@ -204,10 +204,11 @@ class Graph:
             out = np.empty(n, np.float32 if a in (L.PR, L.PPR) else np.uint32)
         st = L.Stats()
         act = None
-        if activity_cap:
+        if activity_cap:  # per-iteration activity (the reference's --activity) into act
             act = np.zeros(activity_cap, np.uint32)
             st.activity = act.ctypes.data_as(C.POINTER(C.c_uint32))
             st.activity_cap = activity_cap
+            o.collect_activity = 1
         rc = L.lib.eeg_run(self._h, a, source, C.byref(o), _ptr(out), C.byref(st))
         L.check(rc, f"eeg_run({algo})")
         d = st.to_dict()
@@ -258,7 +259,7 @@ class Server:
 
     def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
                  ppr_epsilon=0.0, discard_labels=False, lane_wgs_per_cu=0, lanes=0,
-                 plane_engines=False, bfs_engine=True, first_iteration=0):
+                 plane_engines=False, bfs_engine=True, first_iteration=0, ppr_engine=True):
         cfg = L.ServerCfg()
         cfg.label_budget_bytes = label_budget_bytes
         cfg.max_batch = max_batch
@@ -267,7 +268,8 @@ class Server:
         cfg.ppr_epsilon = ppr_epsilon
         cfg.flags = ((L.SERVER_DISCARD_LABELS if discard_labels else 0) |
                      (L.SERVER_PLANE_ENGINES if plane_engines else 0) |
-                     (0 if bfs_engine else L.SERVER_NO_BFS_ENGINE))
+                     (0 if bfs_engine else L.SERVER_NO_BFS_ENGINE) |
+                     (0 if ppr_engine else L.SERVER_NO_PPR_ENGINE))
         cfg.lane_wgs_per_cu = lane_wgs_per_cu
         cfg.lanes = lanes
         cfg.first_iteration = first_iteration
@@ -319,7 +321,15 @@ class Server:
 
 def energy_read(device=0):
     """Device energy accumulator in microjoules, or None if amd-smi is unavailable."""
+    r = energy_read_ts(device)
+    return None if r is None else r[0]
+
+
+def energy_read_ts(device=0):
+    """(microjoules, counter timestamp in ns) of the device energy accumulator, or None.  The
+    counter advances in steps (about once per second on MI355X); the timestamp is that of the
+    last step."""
     uj = C.c_double(0)
     ts = C.c_uint64(0)
     rc = L.lib.eeg_energy_read(device, C.byref(uj), C.byref(ts))
-    return uj.value if rc == L.OK else None
+    return (uj.value, ts.value) if rc == L.OK else None

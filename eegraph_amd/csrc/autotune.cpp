@@ -3,7 +3,8 @@
 // The reference fixes Part_Size = 8 at compile time (include/globals.hpp:30) and takes the
 // variant from --variant (src/argument_parsing.cpp:94-106); this replaces both choices.
 //
-// Search: coordinate descent, variant first at K = 8, then K, then workgroups per CU. Each
+// Search: coordinate descent, variant first at K = 8, then K, then the workgroup size (256, 512
+// or 1024 threads), then workgroups per CU. Each
 // candidate is timed as the median device traversal time over the sample sources after one
 // untimed run. Results are cached per graph shape (log2 N, log2 average degree, log2 degree
 // skew, log2 frontier depth) so that a second graph of the same shape reuses the choice.
@@ -152,12 +153,22 @@ extern "C" eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* 
             consider(o);
         }
     }
-    // 3. resident workgroups per CU
+    // 3. workgroup size (the traversal kernels; PageRank's gather is built for 256 threads)
+    if (algo != EEG_PR && algo != EEG_PPR) {
+        const eeg_opts at = win;
+        for (uint32_t b : {512u, 1024u}) {
+            eeg_opts o = at;
+            o.block = b;
+            consider(o);
+        }
+    }
+    // 4. workgroups per CU, as the same thread counts per CU for every workgroup size
     {
         const eeg_opts at = win;
+        const uint32_t per = (at.block ? at.block : 256u) / 256u;
         for (uint32_t w : {8u, 12u, 16u, 24u, 32u, 48u}) {
             eeg_opts o = at;
-            o.wgs_per_cu = w;
+            o.wgs_per_cu = std::max(1u, w / per);
             consider(o);
         }
     }

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "engine.hpp"
+#include "ppr_plane.hpp"
 
 namespace eeg {
 
@@ -363,28 +364,10 @@ extern "C" eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t*
     double e0 = 0, e1 = 0;
     const bool energy = o.energy && eeg_energy_read(g->device, &e0, nullptr) == EEG_OK;
     if (algo == EEG_PPR) {
-        // one pull iteration per source: planes of several restart vertices were measured
-        // slower (the contribution plane outgrows the caches the single gather relies on)
+        // planes of up to 32 restart vertices: one gather pass per iteration for all of them
         for (uint32_t i = 0; i < nsrc; ++i)
             if (sources[i] >= g->n) return fail(EEG_ERR_INVALID_ARG, "source %u out of range", sources[i]);
-        if (labels_out) EEG_TRY(g->work.out.alloc((size_t)g->n * 4 + 4));
-        for (uint32_t q = 0; q < nsrc; ++q) {
-            eeg_stats ps;
-            memset(&ps, 0, sizeof(ps));
-            EEG_TRY(run_pagerank(g, EEG_PPR, g->h_perm[sources[q]], o, &ps));
-            if (labels_out) {
-                EEG_TRY(unpermute(g, g->work.lab.p, g->work.out.p, g->stream));
-                EEG_HIP(hipMemcpyAsync((float*)labels_out + (uint64_t)q * g->n, g->work.out.p, (size_t)g->n * 4,
-                                       hipMemcpyDeviceToHost, g->stream));
-                EEG_HIP(hipStreamSynchronize(g->stream));
-            }
-            st->iterations = std::max(st->iterations, ps.iterations);
-            st->launches += ps.launches;
-            st->ms_traversal += ps.ms_traversal;
-            st->pr_residual = std::max(st->pr_residual, ps.pr_residual);
-            st->edges_relaxed += ps.edges_relaxed;
-            st->bytes_alg += ps.bytes_alg;
-        }
+        EEG_TRY(run_ppr_batch(g, sources, nsrc, o, (float*)labels_out, st));
         if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
         st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return EEG_OK;

@@ -39,9 +39,9 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Exclusive prefix sum over a workgroup (WG threads); returns the total through *total.
-// Uses `scratch` (WG/WAVE + 1 entries of LDS).  Every thread must call it.
-template <class T>
+// Exclusive prefix sum over a workgroup (BS threads); returns the total through *total.
+// Uses `scratch` (BS/WAVE + 1 entries of LDS).  Every thread must call it.
+template <class T, int BS = WG>
 __device__ __forceinline__ T block_excl_scan(T v, T* scratch, T* total) {
     const int tid = threadIdx.x;
     const int w = tid / WAVE;
@@ -50,27 +50,29 @@ __device__ __forceinline__ T block_excl_scan(T v, T* scratch, T* total) {
     __syncthreads();
     if (tid == 0) {
         T run = 0;
-        for (int i = 0; i < WG / WAVE; ++i) {
+        for (int i = 0; i < BS / WAVE; ++i) {
             T t = scratch[i];
             scratch[i] = run;
             run += t;
         }
-        scratch[WG / WAVE] = run;
+        scratch[BS / WAVE] = run;
     }
     __syncthreads();
     T r = incl - v + scratch[w];
-    *total = scratch[WG / WAVE];
+    *total = scratch[BS / WAVE];
     __syncthreads();
     return r;
 }
 
 // Largest i in [0, n) with S[i] <= t, for ascending S with S[0] <= t, found by the whole
-// workgroup: each round samples WG evenly spaced positions and narrows the interval WG-fold.
+// workgroup of BS threads: each round samples BS evenly spaced positions and narrows the
+// interval BS-fold.
+template <int BS = WG>
 __device__ __forceinline__ uint32_t wg_search(const uint32_t* __restrict__ S, uint32_t n, uint32_t t) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t span = hi - lo;
-        const uint32_t step = (span + WG - 1) / WG;
+        const uint32_t step = (span + BS - 1) / BS;
         const uint64_t i = (uint64_t)lo + (uint64_t)threadIdx.x * step;
         const int pred = (i < hi) && (S[i] <= t);
         const int cnt = __syncthreads_count(pred);

@@ -89,6 +89,17 @@ struct Tigr {
     bool span_built = false;
 };
 
+// Segments of at most 64 in-edges for the many-restart PageRank planes (ppr_plane.hip).
+// Entries are the vertices with in- or out-edges (ascending); entry i owns segments
+// [sscan[i], sscan[i+1]), max(1, ceil(indeg/64)) of them; sdesc describes every segment (first
+// edge, edge count, vertex, spill slot); entries with more than one segment are listed in mlist,
+// and their segments' partial sums go to spill slots from mbase[i].
+struct PprSeg {
+    uint32_t nent = 0, nseg = 0, nmulti = 0, nspill = 0;
+    DevMem ent, sscan, sdesc, mlist, mbase;
+    bool built = false;
+};
+
 // Reusable per-graph traversal state (one query at a time per handle).
 struct Work {
     DevMem lab;         // u32[n] (or f32 ranks), internal vertex order
@@ -137,6 +148,7 @@ struct eeg_graph {
     eeg::Work work;
     eeg::DevMem perm, inv;           // degree order: perm[original] = internal, inv = perm^-1
     eeg::DevMem outdeg;              // u32 out-degrees (built with the PageRank tables)
+    eeg::PprSeg pprseg;              // in-edge segments of the many-restart PageRank planes
     std::vector<uint32_t> h_perm;    // host copy of perm (sources map without a device read)
     uint32_t max_deg = 0, argmax_deg = 0, nonzero_out = 0;
     uint64_t serial = 0;             // unique per handle in this process
@@ -193,5 +205,6 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
                          Work* wk = nullptr, hipStream_t sk = nullptr);
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st, Work* wk = nullptr, hipStream_t sk = nullptr);
+
 
 }  // namespace eeg

@@ -34,5 +34,10 @@ int main(int argc, char** argv) {
         return 1;
     }
     Result r = eegraph_run((uint32_t)algo, arguments, graph);
+    if (arguments.nodeActivity && r.status == EEG_OK) {  // as src/main.cu:171-174 prints them
+        std::cout << "Max active pct: " << utilities::maxActivePct(r.sumLabelsVec, graph.num_nodes) << std::endl;
+        std::cout << "Pct over threshold: " << utilities::pctIterOverThreshold(r.sumLabelsVec, graph.num_nodes, .50)
+                  << std::endl;
+    }
     return (r.status == EEG_OK) ? 0 : 1;
 }

@@ -71,8 +71,10 @@ struct RelaxArgs {
     uint32_t all_first;      // iteration 1 runs every tile (CC: every vertex starts active)
 };
 
-template <int ALG, int K, int MODE>
-__global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
+// BS threads per workgroup (256, 512 or 1024): a tile keeps TILE_EDGES edge slots, so a larger
+// workgroup runs fewer rounds of one edge per lane.
+template <int ALG, int K, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_relax(RelaxArgs args) {
     using R = Rule<ALG>;
     RelaxArgs a = args;
     if (MODE == M_QUEUE && a.dev_it) {
@@ -87,9 +89,9 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
         a.qout = odd ? a.qa : a.qb;
         a.sout = odd ? a.sa : a.sb;
     }
-    constexpr int VPR = WG / K;             // virtual nodes per round
+    constexpr int VPR = BS / K;             // virtual nodes per round
     constexpr int TV = TILE_EDGES / K;      // virtual nodes per tile
-    constexpr int ROUNDS = TILE_EDGES / WG;
+    constexpr int ROUNDS = TILE_EDGES / BS;
     constexpr int EMAX = TV + 2;            // staged entries: every entry has >= 1 vnode
     constexpr int QCAP = (MODE == M_QUEUE) ? TILE_EDGES : 1;
 
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
     __shared__ uint32_t sQ[QCAP];
     __shared__ uint32_t sQn[QCAP];
     __shared__ uint32_t sMisc[8];
-    __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ uint64_t sScan[BS / WAVE + 1];
 
     Ctl* ctl = a.ctl;
     if (ctl->done) return;
@@ -140,20 +142,20 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
 
     constexpr bool SKIP = MODE == M_ASYNC || (MODE == M_FLAGS && ALG != A_CC);
     if (c0 < c1) {
-        uint32_t e_start = SKIP ? 0u : wg_search(a.sin, nE, c0 * TV);
+        uint32_t e_start = SKIP ? 0u : wg_search<BS>(a.sin, nE, c0 * TV);
         for (uint32_t tile = c0; tile < c1; ++tile) {
             if constexpr (SKIP) {
                 const uint64_t bb = a.tblk[tile];
                 const uint32_t blo = (uint32_t)bb, bhi = (uint32_t)(bb >> 32);
                 bool hot = a.it == 1 && a.all_first;
                 if (!hot)
-                    for (uint32_t b = blo + threadIdx.x; b <= bhi; b += WG) hot |= a.vdirty[b] + 1u >= a.tbase + a.it;
+                    for (uint32_t b = blo + threadIdx.x; b <= bhi; b += BS) hot |= a.vdirty[b] + 1u >= a.tbase + a.it;
                 if (!__syncthreads_or(hot)) continue;  // the same answer in every thread
                 e_start = a.tstart[tile];
             }
             const uint32_t t0 = tile * TV;
             const uint32_t t1 = min(t0 + (uint32_t)TV, T);
-            for (int k = threadIdx.x; k < EMAX; k += WG) {
+            for (int k = threadIdx.x; k < EMAX; k += BS) {
                 const uint64_t i = (uint64_t)e_start + k;
                 sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
             }
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                 sMisc[2] = 0;
             }
             __syncthreads();
-            for (int k = threadIdx.x; k < EMAX; k += WG) {
+            for (int k = threadIdx.x; k < EMAX; k += BS) {
                 const uint32_t cur = sS[k];
                 const uint32_t nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
                 if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;  // entries in the tile
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
             __syncthreads();
             const int nent = (int)sMisc[0];
             const int nnext = (int)sMisc[1];
-            for (int k = threadIdx.x; k < nent; k += WG) {
+            for (int k = threadIdx.x; k < nent; k += BS) {
                 const uint32_t v = a.qin[e_start + k];
                 const uint64_t r0 = a.row[v], r1 = a.row[v + 1];
                 const uint32_t lab = a.lab[v];
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                 sLab[k] = lab;
                 st_ent += 1;
             }
-            for (int j = threadIdx.x; j < (int)(t1 - t0); j += WG) {
+            for (int j = threadIdx.x; j < (int)(t1 - t0); j += BS) {
                 const uint32_t t = t0 + j;
                 int lo = 0, hi = nent;
                 while (hi - lo > 1) {
@@ -328,13 +330,13 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                 __syncthreads();
                 const uint32_t cnt = sMisc[2];
                 if (cnt > 0) {
-                    const uint32_t per = (cnt + WG - 1) / WG;
+                    const uint32_t per = (cnt + BS - 1) / BS;
                     const uint32_t b0 = threadIdx.x * per;
                     uint64_t loc = 0;
                     for (uint32_t i = 0; i < per; ++i)
                         if (b0 + i < cnt) loc += sQn[b0 + i];
                     uint64_t tot;
-                    uint64_t pre = block_excl_scan<uint64_t>(loc, sScan, &tot);
+                    uint64_t pre = block_excl_scan<uint64_t, BS>(loc, sScan, &tot);
                     for (uint32_t i = 0; i < per; ++i)
                         if (b0 + i < cnt) {
                             const uint32_t x = sQn[b0 + i];
@@ -350,7 +352,7 @@ __global__ __launch_bounds__(WG) void k_relax(RelaxArgs args) {
                     __syncthreads();
                     const uint32_t bq = sMisc[3], bv = sMisc[4];
                     if ((uint64_t)bq + cnt <= a.cap_out) {
-                        for (uint32_t i = threadIdx.x; i < cnt; i += WG) {
+                        for (uint32_t i = threadIdx.x; i < cnt; i += BS) {
                             a.qout[bq + i] = sQ[i];
                             a.sout[bq + i] = bv + sQn[i];
                         }
@@ -518,32 +520,43 @@ unsigned grid_for(uint64_t work, unsigned cap) {
 
 using KernelFn = void (*)(RelaxArgs);
 
-template <int ALG, int K>
+template <int ALG, int K, int BS>
 KernelFn pick_mode(int mode) {
     switch (mode) {
-        case M_TD: return k_relax<ALG, K, M_TD>;
-        case M_FLAGS: return k_relax<ALG, K, M_FLAGS>;
-        case M_ASYNC: return k_relax<ALG, K, M_ASYNC>;
-        default: return k_relax<ALG, K, M_QUEUE>;
+        case M_TD: return k_relax<ALG, K, M_TD, BS>;
+        case M_FLAGS: return k_relax<ALG, K, M_FLAGS, BS>;
+        case M_ASYNC: return k_relax<ALG, K, M_ASYNC, BS>;
+        default: return k_relax<ALG, K, M_QUEUE, BS>;
     }
 }
-template <int ALG>
+template <int ALG, int BS>
 KernelFn pick_k(uint32_t K, int mode) {
     switch (K) {
-        case 4: return pick_mode<ALG, 4>(mode);
-        case 8: return pick_mode<ALG, 8>(mode);
-        case 16: return pick_mode<ALG, 16>(mode);
-        default: return pick_mode<ALG, 32>(mode);
+        case 4: return pick_mode<ALG, 4, BS>(mode);
+        case 8: return pick_mode<ALG, 8, BS>(mode);
+        case 16: return pick_mode<ALG, 16, BS>(mode);
+        default: return pick_mode<ALG, 32, BS>(mode);
     }
 }
-KernelFn pick_kernel(int alg, uint32_t K, int mode) {
+template <int BS>
+KernelFn pick_alg(int alg, uint32_t K, int mode) {
     switch (alg) {
-        case A_BFS: return pick_k<A_BFS>(K, mode);
-        case A_SSSP: return pick_k<A_SSSP>(K, mode);
-        case A_SSWP: return pick_k<A_SSWP>(K, mode);
-        default: return pick_k<A_CC>(K, mode);
+        case A_BFS: return pick_k<A_BFS, BS>(K, mode);
+        case A_SSSP: return pick_k<A_SSSP, BS>(K, mode);
+        case A_SSWP: return pick_k<A_SSWP, BS>(K, mode);
+        default: return pick_k<A_CC, BS>(K, mode);
     }
 }
+// `block` is 256, 512 or 1024 (checked by the callers)
+KernelFn pick_kernel(int alg, uint32_t K, int mode, uint32_t block) {
+    switch (block) {
+        case 512: return pick_alg<512>(alg, K, mode);
+        case 1024: return pick_alg<1024>(alg, K, mode);
+        default: return pick_alg<256>(alg, K, mode);
+    }
+}
+
+bool block_ok(uint32_t block) { return block == 0 || block == 256 || block == 512 || block == 1024; }
 
 int cu_count(int device) {
     static int cached[64] = {0};
@@ -593,12 +606,13 @@ constexpr uint32_t GRAPH_CHUNK = 6;
 // The executable graph of one chunk for these kernel arguments (buffers, grid, kernel): built
 // on first use and kept in the work state with the last few others, so queries that
 // alternate algorithms reuse theirs.  It holds kernel nodes only.
-eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga, hipStream_t s,
+eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, unsigned block, const RelaxArgs& ga, hipStream_t s,
                        hipGraphExec_t* out) {
-    std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid));
+    std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid) + sizeof(block));
     memcpy(key.data(), &ga, sizeof(RelaxArgs));
     memcpy(key.data() + sizeof(RelaxArgs), &kfn, sizeof(kfn));
     memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn), &grid, sizeof(grid));
+    memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid), &block, sizeof(block));
     for (size_t i = 0; i < w.graphs.size(); ++i) {
         if (w.graphs[i].key == key) {
             Work::GraphEntry hit = std::move(w.graphs[i]);
@@ -627,7 +641,7 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, const RelaxArgs& ga
         hipKernelNodeParams kp{};
         kp.func = (void*)kfn;
         kp.gridDim = dim3(grid);
-        kp.blockDim = dim3(WG);
+        kp.blockDim = dim3(block);
         kp.sharedMemBytes = 0;
         kp.kernelParams = rargs;
         kp.extra = nullptr;
@@ -668,7 +682,8 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     const auto t_host0 = std::chrono::steady_clock::now();
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
-    if (o.block && o.block != WG) return fail(EEG_ERR_INVALID_ARG, "block must be %d", WG);
+    if (!block_ok(o.block)) return fail(EEG_ERR_INVALID_ARG, "block must be 256, 512 or 1024 (got %u)", o.block);
+    const uint32_t block = o.block ? o.block : (uint32_t)WG;
     if (o.wgs_per_cu > 64) return fail(EEG_ERR_INVALID_ARG, "wgs_per_cu must be <= 64");
     int mode;
     switch (o.variant) {
@@ -778,12 +793,15 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     // workgroups per CU of the grid (8 stay resident; more hand out tiles as workgroups finish):
     // measured on RMAT-22 and the 4096^2 grid, flag scans gain from 48, in-place sweeps from 40
     // (32 and 48 each lost on one of the two), queues from 24; topology sweeps stay at 24
-    const uint32_t wpc_static = o.wgs_per_cu ? o.wgs_per_cu : (mode == M_FLAGS ? 48u : mode == M_ASYNC ? 40u : 24u);
-    const uint32_t wpc_queue = o.wgs_per_cu ? o.wgs_per_cu : 24;
+    // (defaults for 256-thread workgroups; larger workgroups keep the same threads per CU)
+    const uint32_t per = block / WG;
+    const uint32_t wpc_static =
+        o.wgs_per_cu ? o.wgs_per_cu : std::max(1u, (mode == M_FLAGS ? 48u : mode == M_ASYNC ? 40u : 24u) / per);
+    const uint32_t wpc_queue = o.wgs_per_cu ? o.wgs_per_cu : std::max(1u, 24u / per);
     const unsigned grid_static =
         (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * wpc_static));
     const unsigned grid_queue = (unsigned)cus * wpc_queue;
-    KernelFn kfn = pick_kernel(alg, K, mode);
+    KernelFn kfn = pick_kernel(alg, K, mode, block);
 
     RelaxArgs ra{};
     ra.row = c.rowp();
@@ -840,7 +858,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
             EEG_HIP(hipEventRecord(eb, s));
         }
         void* args[] = {&ra};
-        EEG_HIP(hipLaunchKernel((const void*)kfn, dim3(grid), dim3(WG), args, 0, s));
+        EEG_HIP(hipLaunchKernel((const void*)kfn, dim3(grid), dim3(block), args, 0, s));
         EEG_TRY(w.event(2 * (size_t)it, &ea));
         EEG_HIP(hipEventRecord(ea, s));
         return EEG_OK;
@@ -852,7 +870,11 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     uint32_t graph_launches = 0;  // nonzero: the replayed-graph path ran
     std::vector<uint32_t> activity;
     if (o.collect_activity) {
-        // one launch at a time so every frontier size can be read back
+        // One launch at a time so every iteration's activity can be read back (the reference's
+        // sumLabels count of active vertices, src/gpu_utils.cu:28-34): with frontier queues the
+        // entries of the frontier the iteration processed (vertices with out-edges), otherwise
+        // the label updates the iteration made.
+        unsigned long long upd_before = 0;
         while (true) {
             EEG_TRY(launch(it));
             EEG_HIP(hipMemcpyAsync(w.host_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -860,7 +882,8 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
             const Ctl& h = w.host_ctl[0];
             if (h.err || h.done) break;
             if (mode == M_QUEUE) activity.push_back((uint32_t)(h.qpack[(it - 1) % 3] & 0xffffffffull));
-            else activity.push_back((uint32_t)h.stat[3]);
+            else activity.push_back((uint32_t)(h.stat[3] - upd_before));
+            upd_before = h.stat[3];
             if (++it > max_iters) {
                 rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
                 break;
@@ -891,7 +914,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         ga.qb = w.q[1].as<uint32_t>();
         ga.sb = w.s[1].as<uint32_t>();
         hipGraphExec_t exec = nullptr;
-        EEG_TRY(chunk_graph(w, kfn, grid_queue, ga, s, &exec));
+        EEG_TRY(chunk_graph(w, kfn, grid_queue, block, ga, s, &exec));
         EEG_TRY(w.poll_events());
         hipEvent_t* poll = w.poll;
         uint32_t replays = 0;

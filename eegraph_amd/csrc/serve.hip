@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "ppr_plane.hpp"
 
 namespace eeg {
 namespace {
@@ -51,7 +52,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 enum TState { T_PENDING = 0, T_RUNNING, T_GATHER, T_DONE, T_CANCELLED };
-enum EngKind { E_BFS = 0, E_SSSP = 1, E_SSWP = 2 };
+enum EngKind { E_BFS = 0, E_SSSP = 1, E_SSWP = 2, E_PPR = 3, E_COUNT = 4 };
 
 struct EvHold {  // one event shared by the answers gathered in one round
     hipEvent_t e = nullptr;
@@ -98,24 +99,34 @@ struct eeg_server;
 namespace eeg {
 namespace {
 
-struct Engine {
+// What every batching engine shares: its thread, its queue of waiting queries, the occupant of
+// each column and the pool of result buffers answers are gathered into.
+struct EngineBase {
     eeg_server* srv = nullptr;
     int kind = E_BFS;
     uint32_t n = 0, K = 8;
-    uint32_t ncols = 0, W = 0, B = 0, qcap = 0;
-    uint32_t R = 0;  // mask record stride: BFS 4W (seen, next, visit, stamp per vertex), planes W
+    uint32_t ncols = 0;
     hipStream_t st = nullptr;
     std::thread th;
     std::condition_variable cv;
     std::deque<TicketP> pending;      // guarded by the server mutex
     std::vector<TicketP> col;         // occupant of each column
-    std::vector<uint8_t> released;    // column given back, reset pending
     uint32_t running = 0;             // columns in T_RUNNING
     bool broken = false;
-    // device state
-    DevMem planes, m1, m3, q[2], s[2], cand, stamp, ctlbuf, stage, spill;
     std::vector<std::unique_ptr<DevMem>> res_all;  // result pool (at most ncols buffers)
     std::vector<DevMem*> res_free;                 // guarded by the server mutex
+    virtual void loop() = 0;
+    virtual ~EngineBase() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+struct Engine : EngineBase {
+    uint32_t W = 0, B = 0, qcap = 0;
+    uint32_t R = 0;  // mask record stride: BFS 4W (seen, next, visit, stamp per vertex), planes W
+    std::vector<uint8_t> released;    // column given back, reset pending
+    // device state
+    DevMem planes, m1, m3, q[2], s[2], cand, stamp, ctlbuf, stage, spill;
     EngCtl* hctl = nullptr;           // pinned, 4 slots
     uint32_t* hstage = nullptr;       // pinned, 4 slots of stage_words
     size_t stage_words = 0;
@@ -133,10 +144,25 @@ struct Engine {
     uint32_t seed_val() const { return kind == E_SSWP ? INF : 0u; }
     eeg_status init(eeg_graph* g);
     eeg_status reset_device();
-    void loop();
+    void loop() override;
     eeg_status launch_iteration(eeg_graph* g, uint32_t j);
     bool gather(std::vector<std::pair<TicketP, uint32_t>>& fin);
-    ~Engine();
+    ~Engine() override;
+};
+
+// Personalised PageRank engine: B columns of rank planes (ppr_plane.hip).  A query occupies a
+// column from admission until its own L1 change drops below the server's epsilon; columns join
+// and leave between iterations, every iteration is one gather pass for all columns in flight.
+struct PprEngine : EngineBase {
+    PprPlane pl;
+    PprCols cur{};                    // restart vertex of every column (PPR_IDLE when free)
+    double* hres = nullptr;           // pinned, 4 slots of 32 residuals
+    hipEvent_t ev[4] = {};
+    uint32_t it = 1;
+    eeg_status init(eeg_graph* g, uint32_t B, float damping);
+    void loop() override;
+    bool gather(std::vector<std::pair<TicketP, uint32_t>>& fin, uint32_t j);
+    ~PprEngine() override;
 };
 
 // A single-query lane: a host thread with its own traversal state and stream, so that several
@@ -164,8 +190,8 @@ struct eeg_server {
     std::unordered_map<uint64_t, eeg::TicketP> tickets;
     uint64_t next_id = 1;
     eeg::Clock::time_point t0;
-    std::unique_ptr<eeg::Engine> eng[3];
-    bool use_eng[3] = {false, false, false};  // algorithms served by a batching engine
+    std::unique_ptr<eeg::EngineBase> eng[eeg::E_COUNT];
+    bool use_eng[eeg::E_COUNT] = {false, false, false, false};  // algorithms served by a batching engine
     bool stop = false;
     // single-query lanes
     std::deque<eeg::TicketP> lane_q;
@@ -186,11 +212,11 @@ namespace eeg {
 namespace {
 
 Engine::~Engine() {
+    if (st) (void)hipStreamSynchronize(st);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     if (hctl) (void)hipHostFree(hctl);
     if (hstage) (void)hipHostFree(hstage);
-    if (st) (void)hipStreamDestroy(st);
 }
 
 __global__ void k_eng_fill32(uint32_t* a, uint64_t n, uint32_t v) {
@@ -310,7 +336,7 @@ eeg_status Engine::launch_iteration(eeg_graph* g, uint32_t j) {
     return launch_plane_iteration(a, K, grid, st, running);
 }
 
-void fail_running(eeg_server* srv, Engine* e, eeg_status why) {
+void fail_running(eeg_server* srv, EngineBase* e, eeg_status why) {
     // caller holds srv->mu
     for (uint32_t c = 0; c < e->ncols; ++c) {
         TicketP& t = e->col[c];
@@ -546,6 +572,187 @@ bool Engine::gather(std::vector<std::pair<TicketP, uint32_t>>& fin) {
     return true;
 }
 
+PprEngine::~PprEngine() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (hres) (void)hipHostFree(hres);
+}
+
+eeg_status PprEngine::init(eeg_graph* g, uint32_t B, float damping) {
+    n = g->n;
+    ncols = B;
+    EEG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto& e : ev) EEG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    EEG_HIP(hipHostMalloc((void**)&hres, 4 * 32 * sizeof(double), hipHostMallocDefault));
+    EEG_TRY(pl.init(g, B, damping, st));
+    for (uint32_t q = 0; q < 32; ++q) cur.src[q] = PPR_IDLE;
+    cur.fresh = 0;
+    col.assign(ncols, nullptr);
+    return EEG_OK;
+}
+
+void PprEngine::loop() {
+    eeg_graph* g = srv->g;
+    (void)hipSetDevice(g->device);
+    const double eps = srv->cfg.ppr_epsilon;
+    constexpr uint32_t kMaxIters = 10000;
+    std::deque<uint32_t> inflight;
+    while (true) {
+        uint32_t fresh = 0, active = 0;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            while (!srv->stop && inflight.empty() && running == 0 && pending.empty()) cv.wait(lk);
+            if (srv->stop) break;
+            const double tnow = srv->now_ms();
+            for (uint32_t c = 0; c < ncols && !pending.empty() && !broken; ++c) {
+                if (col[c]) continue;
+                TicketP tk;
+                while (!pending.empty() && !tk) {
+                    tk = pending.front();
+                    pending.pop_front();
+                    if (tk->cancel) {
+                        tk->state = T_CANCELLED;
+                        tk->status = EEG_ERR_CANCELLED;
+                        srv->cv_done.notify_all();
+                        tk = nullptr;
+                    }
+                }
+                if (!tk) break;
+                col[c] = tk;
+                tk->col = (int)c;
+                tk->state = T_RUNNING;
+                tk->t_start = tnow;
+                tk->first_it = it;
+                cur.src[c] = g->h_perm[tk->source];
+                fresh |= 1u << c;
+                ++running;
+            }
+            active = running;
+            for (uint32_t k = 0; k < ncols; ++k)
+                if (col[k] && col[k]->state == T_RUNNING) col[k]->batch = std::max(col[k]->batch, active);
+        }
+        eeg_status rc = EEG_OK;
+        if (active > 0) {
+            PprCols cols = cur;
+            cols.fresh = fresh;
+            rc = pl.iterate(g, it, cols, st);
+            if (rc == EEG_OK &&
+                (hipMemcpyAsync(hres + (it % 4) * 32, pl.resid.p, (size_t)ncols * 8, hipMemcpyDeviceToHost, st) !=
+                     hipSuccess ||
+                 hipEventRecord(ev[it % 4], st) != hipSuccess))
+                rc = EEG_ERR_DEVICE;
+            if (rc == EEG_OK) inflight.push_back(it++);
+        }
+        if (rc != EEG_OK) {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            fail_running(srv, this, rc);
+            break;
+        }
+        // inspect the older iteration while the newer one runs: a column whose L1 change is
+        // below epsilon has converged; its ranks (the buffer that iteration wrote, intact until
+        // the iteration after the next) are gathered and the column is free for the next query
+        while (!inflight.empty() && (inflight.size() >= 2 || active == 0)) {
+            const uint32_t j = inflight.front();
+            inflight.pop_front();
+            if (hipEventSynchronize(ev[j % 4]) != hipSuccess) {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                fail_running(srv, this, fail(EEG_ERR_DEVICE, "PPR engine: device error"));
+                return;
+            }
+            const double* r = hres + (j % 4) * 32;
+            std::vector<std::pair<TicketP, uint32_t>> fin;
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                const double tnow = srv->now_ms();
+                bool any = false;
+                for (uint32_t c = 0; c < ncols; ++c) {
+                    TicketP& tk = col[c];
+                    if (!tk || tk->state != T_RUNNING || tk->first_it > j) continue;
+                    const bool conv = r[c] < eps;
+                    if (!conv && j - tk->first_it + 1 < kMaxIters) continue;
+                    --running;
+                    if (!conv) tk->err = "PPR residual above epsilon after the iteration limit";
+                    if (srv->discard || tk->cancel || !conv) {
+                        tk->state = T_DONE;
+                        tk->status = conv ? EEG_OK : EEG_ERR_NOT_CONVERGED;
+                        tk->t_done = tnow;
+                        cur.src[c] = PPR_IDLE;
+                        tk = nullptr;
+                        any = true;
+                    } else {
+                        tk->state = T_GATHER;
+                        fin.emplace_back(tk, c);
+                    }
+                }
+                if (any) srv->cv_done.notify_all();
+            }
+            if (!fin.empty() && !gather(fin, j)) return;
+        }
+    }
+}
+
+bool PprEngine::gather(std::vector<std::pair<TicketP, uint32_t>>& fin, uint32_t j) {
+    eeg_graph* g = srv->g;
+    auto evh = std::make_shared<EvHold>();
+    eeg_status rc = hipEventCreateWithFlags(&evh->e, hipEventDisableTiming) == hipSuccess ? EEG_OK : EEG_ERR_DEVICE;
+    std::vector<DevMem*> bufs(fin.size(), nullptr);
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        for (size_t i = 0; i < fin.size(); ++i) {
+            if (!res_free.empty()) {
+                bufs[i] = res_free.back();
+                res_free.pop_back();
+            } else {
+                res_all.emplace_back(new DevMem());
+                bufs[i] = res_all.back().get();
+                bufs[i]->alloc((size_t)n * 4 + 4);
+                if (!bufs[i]->p) bufs[i] = nullptr;
+            }
+        }
+    }
+    for (size_t i = 0; i < fin.size() && rc == EEG_OK; ++i) {
+        TicketP& tk = fin[i].first;
+        const uint32_t c = fin[i].second;
+        if (bufs[i]) {
+            rc = pl.extract(g, j, c, cur.src[c], bufs[i]->as<float>(), st);
+        } else {  // no device memory left for a pooled answer: host copy through the engine stream
+            DevMem tmp;
+            rc = tmp.alloc((size_t)n * 4 + 4);
+            tk->host.resize(n);
+            if (rc == EEG_OK) rc = pl.extract(g, j, c, cur.src[c], tmp.as<float>(), st);
+            if (rc == EEG_OK &&
+                (hipMemcpyAsync(tk->host.data(), tmp.p, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipStreamSynchronize(st) != hipSuccess))
+                rc = EEG_ERR_DEVICE;
+        }
+    }
+    if (rc == EEG_OK && hipEventRecord(evh->e, st) != hipSuccess) rc = EEG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    const double tnow = srv->now_ms();
+    for (size_t i = 0; i < fin.size(); ++i) {
+        TicketP& tk = fin[i].first;
+        const uint32_t c = fin[i].second;
+        tk->state = T_DONE;
+        tk->status = rc == EEG_OK ? EEG_OK : fail(EEG_ERR_DEVICE, "gathering an answer failed");
+        tk->t_done = tnow;
+        tk->res = bufs[i];
+        tk->ev = evh;
+        if (tk->cancel && tk->res) {
+            res_free.push_back(tk->res);
+            tk->res = nullptr;
+        }
+        cur.src[c] = PPR_IDLE;  // the column takes a new query from the next iteration on
+        col[c] = nullptr;
+    }
+    srv->cv_done.notify_all();
+    if (rc != EEG_OK) {
+        fail_running(srv, this, EEG_ERR_DEVICE);
+        return false;
+    }
+    return true;
+}
+
 // Tables a query reads, built once (under the graph's lock) before any lane uses them.
 eeg_status prepare_for(eeg_server* srv, uint32_t algo) {
     eeg_graph* g = srv->g;
@@ -636,7 +843,8 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32");
     if (c.damping == 0.0f) c.damping = 0.85f;
     if (c.ppr_epsilon == 0.0) c.ppr_epsilon = 1e-7;
-    const uint32_t known = EEG_SERVER_DISCARD_LABELS | EEG_SERVER_PLANE_ENGINES | EEG_SERVER_NO_BFS_ENGINE;
+    const uint32_t known =
+        EEG_SERVER_DISCARD_LABELS | EEG_SERVER_PLANE_ENGINES | EEG_SERVER_NO_BFS_ENGINE | EEG_SERVER_NO_PPR_ENGINE;
     if (c.flags & ~known) return fail(EEG_ERR_INVALID_ARG, "unknown flags 0x%x", c.flags);
     if (c.lanes > 64) return fail(EEG_ERR_INVALID_ARG, "lanes must be <= 64");
     DeviceGuard dg;
@@ -675,15 +883,38 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     for (int k = 0; k < 3; ++k) srv->use_eng[k] = use[k];
     for (int k = 0; k < 3 && ncols > 0; ++k) {
         if (!use[k]) continue;
-        srv->eng[k].reset(new (std::nothrow) Engine());
-        if (!srv->eng[k]) return fail(EEG_ERR_OOM, "host allocation failed");
-        Engine& e = *srv->eng[k];
-        e.srv = srv.get();
-        e.kind = k;
-        e.K = K;
-        e.ncols = ncols;
-        if (c.first_iteration) e.it = c.first_iteration;
-        EEG_TRY(e.init(g));
+        Engine* e = new (std::nothrow) Engine();
+        if (!e) return fail(EEG_ERR_OOM, "host allocation failed");
+        srv->eng[k].reset(e);
+        e->srv = srv.get();
+        e->kind = k;
+        e->K = K;
+        e->ncols = ncols;
+        if (c.first_iteration) e->it = c.first_iteration;
+        EEG_TRY(e->init(g));
+    }
+    // PPR engine: rank planes of 32 columns, or 16 when 32 do not fit a third of the free memory
+    if ((c.flags & EEG_SERVER_NO_PPR_ENGINE) == 0 && n > 0) {
+        {
+            std::lock_guard<std::mutex> lk(g->mu);
+            EEG_TRY(ensure_in(g));
+            EEG_TRY(ensure_ppr_seg(g));
+        }
+        size_t fr = 0, tot = 0;
+        EEG_HIP(hipMemGetInfo(&fr, &tot));
+        const uint32_t nsp = g->pprseg.nspill;
+        const uint32_t B = PprPlane::bytes_for(n, 32, nsp) <= fr / 3 ? 32u
+                           : PprPlane::bytes_for(n, 16, nsp) <= fr / 3 ? 16u : 0u;
+        if (B) {
+            PprEngine* e = new (std::nothrow) PprEngine();
+            if (!e) return fail(EEG_ERR_OOM, "host allocation failed");
+            srv->eng[E_PPR].reset(e);
+            e->srv = srv.get();
+            e->kind = E_PPR;
+            e->K = K;
+            EEG_TRY(e->init(g, B, c.damping));
+            srv->use_eng[E_PPR] = true;
+        }
     }
     EEG_HIP(hipStreamCreateWithFlags(&srv->io_st, hipStreamNonBlocking));
     EEG_TRY(srv->io_buf.alloc((size_t)n * 4 + 4));
@@ -693,7 +924,7 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         if (!srv->lanes.back()) return fail(EEG_ERR_OOM, "host allocation failed");
         EEG_HIP(hipStreamCreateWithFlags(&srv->lanes.back()->st, hipStreamNonBlocking));
     }
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < E_COUNT; ++k)
         if (srv->eng[k]) srv->eng[k]->th = std::thread([e = srv->eng[k].get()] { e->loop(); });
     for (auto& ln : srv->lanes) ln->th = std::thread([s = srv.get(), l = ln.get()] { lane_loop(s, l); });
     *out = srv.release();
@@ -715,9 +946,13 @@ extern "C" eeg_status eeg_server_submit(eeg_server* s, uint32_t algo, uint32_t s
     if (s->stop) return fail(EEG_ERR_INVALID_ARG, "server is shutting down");
     tk->id = s->next_id++;
     tk->t_submit = s->now_ms();
-    const int e = algo == EEG_BFS ? E_BFS : algo == EEG_SSSP ? E_SSSP : algo == EEG_SSWP ? E_SSWP : -1;
+    const int e = algo == EEG_BFS    ? E_BFS
+                  : algo == EEG_SSSP ? E_SSSP
+                  : algo == EEG_SSWP ? E_SSWP
+                  : algo == EEG_PPR  ? E_PPR
+                                     : -1;
     if (e >= 0 && s->use_eng[e]) {
-        Engine* eng = s->eng[e].get();
+        EngineBase* eng = s->eng[e].get();
         if (!eng) return fail(EEG_ERR_REJECTED, "label budget holds no column for this algorithm");
         if (eng->broken) return fail(EEG_ERR_DEVICE, "serving engine stopped after a device error");
         tk->engine = e;
@@ -759,7 +994,7 @@ extern "C" eeg_status eeg_server_wait(eeg_server* s, uint64_t ticket, void* labe
         res->t_done_ms = tk->t_done;
     }
     if (tk->state == T_CANCELLED) return fail(EEG_ERR_CANCELLED, "ticket %llu was cancelled", (unsigned long long)ticket);
-    Engine* e = tk->engine >= 0 ? s->eng[tk->engine].get() : nullptr;
+    EngineBase* e = tk->engine >= 0 ? s->eng[tk->engine].get() : nullptr;
     DevMem* buf = tk->res;
     tk->res = nullptr;
     if (tk->status != EEG_OK) {

@@ -82,7 +82,9 @@ typedef struct eeg_server eeg_server; /* opaque: request loop over one eeg_graph
 typedef struct eeg_opts {
     uint32_t variant;          /* eeg_variant, default EEG_ASYNC_PUSH_TD like the reference */
     uint32_t K;                /* virtual-node degree bound: 4, 8, 16 or 32 (0 = 8) */
-    uint32_t block;            /* workgroup size: 256 (0 = 256) */
+    uint32_t block;            /* workgroup size of the BFS/SSSP/SSWP/CC relaxation kernels:
+                                  256, 512 or 1024 threads (0 = 256; the reference launches 512,
+                                  src/eegraph.cu:80-81); PageRank and the batched passes use 256 */
     uint32_t frontier;         /* eeg_frontier_kind, used by EEG_SYNC_PUSH_DD */
     uint32_t max_iters;        /* 0 = no limit beyond the algorithm's own bound */
     uint32_t collect_activity; /* fill stats->activity with the per-iteration frontier size */
@@ -239,7 +241,8 @@ eeg_status eeg_run_packed(eeg_graph* const* graphs, uint32_t ngraphs, uint32_t a
 /* ---------------- autotuner ----------------
  * No counterpart in the reference, which fixes Part_Size = 8 (include/globals.hpp:30) and
  * takes the variant from --variant (src/argument_parsing.cpp:94-106).  Picks the variant and
- * frontier kind, K and wgs_per_cu for this graph and algorithm by timing short runs from the
+ * frontier kind, K, the workgroup size and wgs_per_cu for this graph and algorithm by timing
+ * short runs from the
  * sample sources (NULL/0 = the max-out-degree vertex; at most 16 are used).  damping, acc,
  * epsilon and max_iters are read from *best and kept; the other fields are overwritten with
  * the winner.  *best_ms receives its median traversal time.  Choices are cached per graph
@@ -249,14 +252,14 @@ eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sour
 
 /* ---------------- serving ----------------
  * No counterpart in the reference (one query per process, src/main.cu:101-121).  A request
- * loop over one device-resident graph.  Engine queries (BFS; SSSP and SSWP with
- * EEG_SERVER_PLANE_ENGINES) take a column of their engine's label plane when one is free
- * (admission by label memory) and join the next iteration of the traversal already running
+ * loop over one device-resident graph.  Engine queries (BFS; PPR; SSSP and SSWP with
+ * EEG_SERVER_PLANE_ENGINES) take a column of their engine's planes when one is free
+ * (admission by label memory) and join the next iteration of the pass already running
  * (continuous batching); the column is given back at convergence, the answer kept in a result
  * buffer until it is read.  Other queries run on `lanes` concurrent single-query lanes whose
- * launches overlap on the device.  Every answer equals eeg_run's for the same query.  The
- * graph must outlive the server; eeg_run calls on the same graph from other threads are
- * allowed. */
+ * launches overlap on the device.  Every answer equals eeg_run's for the same query (PPR:
+ * within the stated tolerance, the same iteration summed in another fixed order).  The graph
+ * must outlive the server; eeg_run calls on the same graph from other threads are allowed. */
 typedef struct eeg_server_cfg {
     uint64_t label_budget_bytes; /* HBM for the engines' label planes (0 = min(free/4, 48 GiB));
                                     split evenly over the BFS, SSSP and SSWP engines, each
@@ -269,7 +272,7 @@ typedef struct eeg_server_cfg {
     uint32_t lane_wgs_per_cu;    /* resident workgroups per CU of a single-query lane (0 = 16),
                                     so its kernels leave room for other lanes and the engines */
     uint32_t lanes;              /* single-query lanes (0 = 2, <= 64) */
-    uint32_t first_iteration;    /* testing: the engines' iteration counter at start (0 = 1);
+    uint32_t first_iteration;    /* testing: the traversal engines' iteration counter at start (0 = 1);
                                     from 0xf0000000 on an engine admits nothing until it drains,
                                     then resets its columns and restarts the counter at 1 */
     uint32_t reserved[4];
@@ -281,6 +284,9 @@ typedef struct eeg_server_cfg {
 #define EEG_SERVER_PLANE_ENGINES 2u
 /* BFS goes through lanes instead of the multi-source BFS engine. */
 #define EEG_SERVER_NO_BFS_ENGINE 4u
+/* PPR goes through lanes instead of the PPR engine (rank planes of 16 or 32 restart vertices,
+ * one gather pass per iteration for every query in flight). */
+#define EEG_SERVER_NO_PPR_ENGINE 8u
 typedef struct eeg_query_result {
     uint32_t algo;
     uint32_t source;

@@ -19,17 +19,48 @@ static double now_s(void) {
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Counting sort, stable by input order.  In parallel: every thread owns a range of sources
+ * holding about m/T edges and scans the whole list in input order, placing only its own
+ * sources' edges, so each row keeps input order whatever the thread count. */
 void orc_csr_build(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                    const uint32_t* w, uint64_t* row, uint32_t* col, uint32_t* wout) {
     memset(row, 0, sizeof(uint64_t) * ((size_t)n + 1));
-    for (uint64_t i = 0; i < m; ++i) row[src[i] + 1]++;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)m; ++i) {
+#pragma omp atomic
+        row[src[i] + 1]++;
+    }
     for (uint32_t v = 0; v < n; ++v) row[v + 1] += row[v];
     uint64_t* cur = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
     memcpy(cur, row, sizeof(uint64_t) * ((size_t)n + 1));
-    for (uint64_t i = 0; i < m; ++i) {
-        const uint64_t p = cur[src[i]]++;
-        col[p] = dst[i];
-        if (wout) wout[p] = w ? w[i] : 1u;
+#pragma omp parallel
+    {
+        const uint64_t T = (uint64_t)omp_get_num_threads(), t = (uint64_t)omp_get_thread_num();
+        /* first source whose row starts at or after edge k (sources [lo, hi) are this thread's) */
+#define ORC_FIRST_AT(k, out)                         \
+        do {                                         \
+            uint32_t a_ = 0, b_ = n;                 \
+            while (a_ < b_) {                        \
+                const uint32_t c_ = a_ + (b_ - a_) / 2; \
+                if (row[c_] < (k)) a_ = c_ + 1; else b_ = c_; \
+            }                                        \
+            (out) = a_;                              \
+        } while (0)
+        uint32_t lo, hi;
+        ORC_FIRST_AT(m * t / T, lo);
+        ORC_FIRST_AT(m * (t + 1) / T, hi);
+        if (t == 0) lo = 0;
+        if (t + 1 == T) hi = n;
+#undef ORC_FIRST_AT
+        if (lo < hi) {
+            for (uint64_t i = 0; i < m; ++i) {
+                const uint32_t s = src[i];
+                if (s < lo || s >= hi) continue;
+                const uint64_t p = cur[s]++;
+                col[p] = dst[i];
+                if (wout) wout[p] = w ? w[i] : 1u;
+            }
+        }
     }
     free(cur);
 }
@@ -253,6 +284,103 @@ void orc_widest(uint32_t n, const uint64_t* row, const uint32_t* col, const uint
     free(h.a);
 }
 
+/* Growable vertex list of one bucket. */
+typedef struct {
+    uint32_t* a;
+    size_t n, cap;
+} Bucket;
+static void bucket_push(Bucket* b, uint32_t v) {
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 64;
+        b->a = (uint32_t*)realloc(b->a, sizeof(uint32_t) * b->cap);
+    }
+    b->a[b->n++] = v;
+}
+
+/* Dial's bucket queue (R. B. Dial, "Algorithm 360: shortest-path forest with topological
+ * ordering", CACM 12(11), 1969) for integer weights in [0, wmax]: wmax+1 circular buckets
+ * indexed by distance mod (wmax+1); the smallest tentative distance is the current bucket, so
+ * each vertex is settled when its bucket comes round.  Same answer as orc_dijkstra in
+ * O(m + max distance) instead of O(m log n). */
+int orc_dial(uint32_t n, const uint64_t* row, const uint32_t* col, const uint32_t* w, uint32_t wmax,
+             uint32_t source, uint32_t* out) {
+    if (w) {
+        for (uint64_t e = 0; e < row[n]; ++e)
+            if (w[e] > wmax) return -1;
+    }
+    const size_t nb = (size_t)wmax + 1;
+    Bucket* bk = (Bucket*)calloc(nb, sizeof(Bucket));
+    uint64_t* dist = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+    for (uint32_t v = 0; v < n; ++v) dist[v] = UINT64_MAX;
+    dist[source] = 0;
+    bucket_push(&bk[0], source);
+    uint64_t queued = 1, cur = 0;
+    while (queued) {
+        Bucket* b = &bk[cur % nb];
+        for (size_t i = 0; i < b->n; ++i) {  /* weight-0 edges append to this bucket */
+            const uint32_t v = b->a[i];
+            --queued;
+            if (dist[v] != cur) continue;  /* settled earlier at a smaller distance */
+            for (uint64_t e = row[v]; e < row[v + 1]; ++e) {
+                const uint64_t nd = cur + (w ? w[e] : 1u);
+                if (nd < dist[col[e]]) {
+                    dist[col[e]] = nd;
+                    bucket_push(&bk[nd % nb], col[e]);
+                    ++queued;
+                }
+            }
+        }
+        b->n = 0;
+        ++cur;
+    }
+    for (uint32_t v = 0; v < n; ++v) out[v] = dist[v] >= ORC_INF ? ORC_INF : (uint32_t)dist[v];
+    for (size_t i = 0; i < nb; ++i) free(bk[i].a);
+    free(bk);
+    free(dist);
+    return 0;
+}
+
+/* Widest (max-min) paths with a bucket per label value, for integer weights in [0, wmax]: a
+ * label only moves down along a path, so buckets are settled from wmax down to 1; the source
+ * holds ORC_INF and gives each out-neighbour its edge weight.  Same answer as orc_widest. */
+int orc_dial_widest(uint32_t n, const uint64_t* row, const uint32_t* col, const uint32_t* w,
+                    uint32_t wmax, uint32_t source, uint32_t* out) {
+    if (w) {
+        for (uint64_t e = 0; e < row[n]; ++e)
+            if (w[e] > wmax) return -1;
+    }
+    const size_t nb = (size_t)wmax + 1;
+    Bucket* bk = (Bucket*)calloc(nb, sizeof(Bucket));
+    for (uint32_t v = 0; v < n; ++v) out[v] = 0;
+    out[source] = ORC_INF;
+    for (uint64_t e = row[source]; e < row[source + 1]; ++e) {
+        const uint32_t c = w ? w[e] : 1u;
+        if (c > out[col[e]]) {
+            out[col[e]] = c;
+            bucket_push(&bk[c], col[e]);
+        }
+    }
+    for (size_t lv = wmax; lv >= 1; --lv) {
+        Bucket* b = &bk[lv];
+        for (size_t i = 0; i < b->n; ++i) {  /* edges of weight >= lv append to this bucket */
+            const uint32_t v = b->a[i];
+            if (out[v] != lv) continue;  /* raised to a higher bucket meanwhile */
+            for (uint64_t e = row[v]; e < row[v + 1]; ++e) {
+                const uint32_t we = w ? w[e] : 1u;
+                const uint32_t c = we < lv ? we : (uint32_t)lv;
+                if (c > out[col[e]]) {
+                    out[col[e]] = c;
+                    bucket_push(&bk[c], col[e]);
+                }
+            }
+        }
+        b->n = 0;
+    }
+    for (size_t i = 0; i < nb; ++i) free(bk[i].a);
+    free(bk);
+    return 0;
+}
+
 static uint32_t uf_find(uint32_t* p, uint32_t x) {
     while (p[x] != x) {
         p[x] = p[p[x]];
@@ -353,12 +481,13 @@ uint32_t orc_pagerank(uint32_t n, const uint64_t* row, const uint32_t* col, doub
 
 uint32_t orc_pagerank_pull(uint32_t n, const uint64_t* in_row, const uint32_t* in_col,
                            const uint32_t* outdeg, double d, double eps, uint32_t max_iter,
-                           int threads, double* x, double* residual) {
+                           int threads, int personalised, uint32_t source, double* x,
+                           double* residual) {
     double* nx = (double*)malloc(sizeof(double) * ((size_t)n + 1));
     double* c = (double*)malloc(sizeof(double) * ((size_t)n + 1));
     if (threads > 0) omp_set_num_threads(threads);
-    const double base = (1.0 - d) / n;
-    for (uint32_t v = 0; v < n; ++v) x[v] = 1.0 / n;
+    const double base = personalised ? 0.0 : (1.0 - d) / n;
+    for (uint32_t v = 0; v < n; ++v) x[v] = personalised ? (v == source ? 1.0 : 0.0) : 1.0 / n;
     uint32_t it = 0;
     double res = 0;
     while (it < max_iter) {
@@ -367,7 +496,7 @@ uint32_t orc_pagerank_pull(uint32_t n, const uint64_t* in_row, const uint32_t* i
         res = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : res)
         for (int64_t v = 0; v < (int64_t)n; ++v) {
-            double s = base;
+            double s = (personalised && (uint32_t)v == source) ? 1.0 - d : base;
             for (uint64_t e = in_row[v]; e < in_row[v + 1]; ++e) s += c[in_col[e]];
             nx[v] = s;
             res += fabs(s - x[v]);

@@ -52,6 +52,13 @@ void orc_dijkstra(uint32_t n, const uint64_t* row, const uint32_t* col, const ui
                   uint32_t source, uint32_t* out);
 void orc_widest(uint32_t n, const uint64_t* row, const uint32_t* col, const uint32_t* w,
                 uint32_t source, uint32_t* out);
+/* The same answers as orc_dijkstra / orc_widest for integer weights in [0, wmax] with bucket
+ * queues (Dial 1969), linear in m plus the largest label: for the largest graphs.  Return -1
+ * (out untouched) if a weight exceeds wmax. */
+int orc_dial(uint32_t n, const uint64_t* row, const uint32_t* col, const uint32_t* w, uint32_t wmax,
+             uint32_t source, uint32_t* out);
+int orc_dial_widest(uint32_t n, const uint64_t* row, const uint32_t* col, const uint32_t* w,
+                    uint32_t wmax, uint32_t source, uint32_t* out);
 /* Weakly connected components: label = minimum vertex id in the component. */
 void orc_wcc(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, uint32_t* out);
 
@@ -67,14 +74,15 @@ uint32_t orc_pagerank(uint32_t n, const uint64_t* row, const uint32_t* col, doub
                       uint32_t max_iter, int personalised, uint32_t source, double* x,
                       double* residual);
 
-/* The same FP64 Jacobi iteration (not personalised) in pull form over the in-edge CSR
+/* The same FP64 Jacobi iteration (also personalised) in pull form over the in-edge CSR
  * (in_row/in_col: sources of each vertex's in-edges; outdeg: out-degree of every vertex),
  * parallel over destinations with OpenMP (`threads` <= 0: the OpenMP default).  Each vertex's
  * sum is taken in in-edge order by one thread, so the ranks do not depend on the thread
  * count.  For large graphs, where orc_pagerank would take minutes. */
 uint32_t orc_pagerank_pull(uint32_t n, const uint64_t* in_row, const uint32_t* in_col,
                            const uint32_t* outdeg, double d, double eps, uint32_t max_iter,
-                           int threads, double* x, double* residual);
+                           int threads, int personalised, uint32_t source, double* x,
+                           double* residual);
 
 /* Serial delta-push PageRank following src/pr.cu:171-225 (value += delta, push d*delta/deg)
  * with D6 fixed (each vertex consumes its delta once) and in the normalised scale

@@ -45,7 +45,9 @@ _sig("orc_brute", C.c_int, C.c_int, C.c_uint32, C.c_uint64, _vp, _vp, _vp, C.c_u
 _sig("orc_pagerank", C.c_uint32, C.c_uint32, _vp, _vp, C.c_double, C.c_double, C.c_uint32, C.c_int,
      C.c_uint32, _vp, C.POINTER(C.c_double))
 _sig("orc_pagerank_pull", C.c_uint32, C.c_uint32, _vp, _vp, _vp, C.c_double, C.c_double, C.c_uint32,
-     C.c_int, _vp, C.POINTER(C.c_double))
+     C.c_int, C.c_int, C.c_uint32, _vp, C.POINTER(C.c_double))
+_sig("orc_dial", C.c_int, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp)
+_sig("orc_dial_widest", C.c_int, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp)
 _sig("orc_pr_delta_push", C.c_uint32, C.c_uint32, _vp, _vp, C.c_float, C.c_float, _vp, C.c_double,
      C.POINTER(C.c_int))
 
@@ -137,13 +139,34 @@ def pagerank(n, row, col, d=0.85, eps=1e-12, max_iter=10000, personalised=False,
     return x, it, res.value
 
 
-def pagerank_pull(n, in_row, in_col, outdeg, d=0.85, eps=1e-12, max_iter=10000, threads=0):
+def pagerank_pull(n, in_row, in_col, outdeg, d=0.85, eps=1e-12, max_iter=10000, threads=0,
+                  personalised=False, source=0):
     """FP64 Jacobi PageRank over the in-edge CSR, OpenMP over destinations."""
     x = np.empty(n, np.float64)
     res = C.c_double(0)
     it = _lib.orc_pagerank_pull(n, _p(in_row), _p(in_col), _p(outdeg), d, eps, max_iter, threads,
-                                _p(x), C.byref(res))
+                                1 if personalised else 0, source, _p(x), C.byref(res))
     return x, it, res.value
+
+
+def dial(n, row, col, w, source, wmax=None):
+    """Dijkstra's answer with Dial's bucket queue (integer weights <= wmax)."""
+    if wmax is None:
+        wmax = int(w.max()) if w is not None and len(w) else 1
+    out = np.empty(n, np.uint32)
+    if _lib.orc_dial(n, _p(row), _p(col), _p(w), wmax, source, _p(out)) != 0:
+        raise ValueError("a weight exceeds wmax")
+    return out
+
+
+def dial_widest(n, row, col, w, source, wmax=None):
+    """Max-min Dijkstra's answer with a bucket per label value (integer weights <= wmax)."""
+    if wmax is None:
+        wmax = int(w.max()) if w is not None and len(w) else 1
+    out = np.empty(n, np.uint32)
+    if _lib.orc_dial_widest(n, _p(row), _p(col), _p(w), wmax, source, _p(out)) != 0:
+        raise ValueError("a weight exceeds wmax")
+    return out
 
 
 def pr_delta_push(n, row, col, d=0.85, thr=1e-10, time_cap_s=0.0):

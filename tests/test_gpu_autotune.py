@@ -25,7 +25,9 @@ def test_autotune_integer(rmat14, alg):
     o, ms = g.autotune(alg, [src, 7, 100])
     assert ms > 0 and np.isfinite(ms)
     assert o.K in (4, 8, 16, 32)
-    assert o.wgs_per_cu in (0, 8, 12, 16, 24, 32, 48)
+    assert o.block in (0, 256, 512, 1024)  # the workgroup size is one of the searched choices
+    per = max(o.block, 256) // 256
+    assert o.wgs_per_cu in (0,) + tuple(max(1, x // per) for x in (8, 12, 16, 24, 32, 48))
     assert o.variant in (eg.VARIANTS["async_push_td"], eg.VARIANTS["async_push_dd"],
                          eg.VARIANTS["sync_push_dd"])
     tuned, _ = g.run(alg, src, opts=o)
@@ -36,7 +38,8 @@ def test_autotune_integer(rmat14, alg):
     assert np.array_equal(tuned, ref)
     # a second call on the same shape is answered from the cache: same choice
     o2, ms2 = g.autotune(alg, [src, 7, 100])
-    assert (o2.variant, o2.frontier, o2.K, o2.wgs_per_cu) == (o.variant, o.frontier, o.K, o.wgs_per_cu)
+    assert (o2.variant, o2.frontier, o2.K, o2.block, o2.wgs_per_cu) == (o.variant, o.frontier, o.K, o.block,
+                                                                        o.wgs_per_cu)
     assert ms2 > 0
 
 

@@ -120,3 +120,32 @@ def test_batch_grid_size_does_not_change_answers(rmat13, alg, nsrc):
         assert np.array_equal(out, ref), (alg, nsrc, wpc)
     single, _ = g.run(alg, int(srcs[-1]), variant="sync_push_dd", K=8)
     assert np.array_equal(ref[-1], single)
+
+
+@pytest.mark.parametrize("nsrc", [5, 16, 40])
+def test_batched_ppr_planes_against_single_and_fp64(rmat13, nsrc):
+    """Personalised PageRank for many restart vertices in planes of 16 or 32 columns (one
+    gather pass per iteration for all of them): every row within L1 1e-6 / max 1e-7 of the FP64
+    oracle and of its single-query run.  Sources include the hub, a vertex without out-edges, a
+    vertex without any edge, and repeats."""
+    n, s, d, w, g = rmat13
+    row, col, _ = O.csr(n, s, d, w)
+    outdeg = np.diff(row)
+    indeg = np.bincount(d, minlength=n)
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.integers(0, n, nsrc).astype(np.uint32)
+    srcs[0] = g.info()["argmax_out_degree"]
+    srcs[1] = int(np.flatnonzero((outdeg == 0) & (indeg > 0))[0])
+    iso = np.flatnonzero((outdeg == 0) & (indeg == 0))
+    if len(iso):
+        srcs[2] = int(iso[0])
+    srcs[-1] = srcs[0]
+    lab, st = g.run_batch("ppr", srcs)
+    assert lab.shape == (nsrc, n) and lab.dtype == np.float32
+    for q, x in enumerate(srcs):
+        ref, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(x))
+        err = np.abs(lab[q].astype(np.float64) - ref)
+        assert err.sum() < 1e-6 and err.max() < 1e-7, (q, int(x), err.sum(), err.max())
+        if q < 4:
+            single, _ = g.run("ppr", int(x), variant="sync_push_dd")
+            assert np.abs(lab[q].astype(np.float64) - single.astype(np.float64)).sum() < 1e-6

@@ -43,7 +43,7 @@ def _check(g, alg, src, lab, n, row, col, singles):
 
 
 MODES = {"default": {}, "planes": {"plane_engines": True},
-         "lanes": {"bfs_engine": False, "lanes": 3}}
+         "lanes": {"bfs_engine": False, "ppr_engine": False, "lanes": 3}}
 
 
 @pytest.mark.parametrize("mode", list(MODES))
@@ -155,3 +155,45 @@ def test_discard_mode_and_errors(rmat12):
         with pytest.raises(eg.EEGError) as e:
             srv.submit("bfs", 0)
         assert e.value.status == L.ERR_REJECTED
+
+
+def test_engine_counter_restarts_near_its_limit(rmat12):
+    """Engines whose iteration counter starts just below the restart limit (0xf0000000): the
+    first queries run past it, further admissions wait until the engine drains, then every
+    column is reset and the counter restarts at 1.  Answers stay equal to eeg_run's across the
+    restart, for the BFS engine (levels stored as counter + depth) and the plane engines."""
+    n, s, d, w, g, row, col = rmat12
+    rng = np.random.default_rng(13)
+    srcs = rng.integers(0, n, size=48)
+    srcs[::5] = g.info()["argmax_out_degree"]
+    singles = {}
+    with eg.Server(g, max_batch=8, plane_engines=True, first_iteration=0xf0000000 - 3) as srv:
+        for alg in ("bfs", "sssp", "sswp"):
+            tk = [srv.submit(alg, int(x)) for x in srcs]
+            for t, x in zip(tk, srcs):
+                lab, r = srv.wait(t)
+                assert r["status"] == L.OK
+                _check(g, alg, int(x), lab, n, row, col, singles)
+
+
+def test_ppr_engine_columns_join_and_leave(rmat12):
+    """More PPR queries than the engine has columns, submitted while others run: each answer
+    within L1 1e-6 of its single-query run and of the FP64 oracle; queries share iterations."""
+    n, s, d, w, g, row, col = rmat12
+    rng = np.random.default_rng(17)
+    srcs = [int(x) for x in rng.integers(0, n, size=70)]
+    srcs[::9] = [g.info()["argmax_out_degree"]] * len(srcs[::9])
+    singles = {}
+    with eg.Server(g) as srv:
+        tk = []
+        for i, x in enumerate(srcs):
+            tk.append(srv.submit("ppr", x))
+            if i % 10 == 0:
+                time.sleep(0.002)
+        res = [srv.wait(t, algo="ppr") for t in tk]
+    shared = 0
+    for (lab, r), x in zip(res, srcs):
+        assert r["status"] == L.OK
+        _check(g, "ppr", x, lab, n, row, col, singles)
+        shared += r["batch_size"] > 1
+    assert shared > 0

@@ -242,3 +242,19 @@ def test_grid_size_does_not_change_labels(rmat14, refs14, alg):
         for wpc in (0, 1, 40, 64):
             lab, _ = g.run(alg, src, variant=variant, frontier=frontier, K=8, wgs_per_cu=wpc)
             assert np.array_equal(lab, refs[alg]), f"{alg} {variant} {frontier} wgs_per_cu={wpc}"
+
+
+@pytest.mark.parametrize("block", [512, 1024])
+@pytest.mark.parametrize("variant", VARIANTS + ["flags"])
+@pytest.mark.parametrize("K", [4, 32])
+def test_workgroup_sizes_bit_exact(rmat14, refs14, block, variant, K):
+    """The relaxation kernels at 512 and 1024 threads per workgroup (the autotuner's block
+    choices; the reference launches 512, src/eegraph.cu:80-81) give the oracle's labels."""
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    kw = dict(variant="sync_push_dd", frontier="flags") if variant == "flags" else dict(variant=variant)
+    for alg in ALG:
+        o = eg.make_opts(K=K, **kw)
+        o.block = block
+        lab, st = g.run(alg, src, opts=o)
+        assert np.array_equal(lab, refs[alg]), f"{alg} {variant} K={K} block={block}"

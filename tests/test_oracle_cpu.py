@@ -189,3 +189,68 @@ def test_pagerank_pull_equals_push_form():
         xq, itq, _ = O.pagerank_pull(n, irow, icol, outdeg, 0.85, 1e-12, threads=threads)
         assert itq == itp
         assert np.abs(xq - xp).max() < 1e-15 and np.abs(xq - xp).sum() < 1e-12
+
+
+def test_personalised_pull_equals_push_form():
+    """The personalised FP64 pull (one-hot restart) equals the serial personalised push form for
+    several restart vertices and thread counts; a different restart vertex gives other ranks."""
+    s, d, w = eg.gen_rmat(12, 8, seed=3)
+    n = 1 << 12
+    row, col, _ = O.csr(n, s, d)
+    irow, icol, _ = O.csr(n, d, s)
+    outdeg = np.diff(row).astype(np.uint32)
+    last = None
+    for src in (0, 77, 1000):
+        xp, itp, _ = O.pagerank(n, row, col, 0.85, 1e-12, personalised=True, source=src)
+        for threads in (1, 4):
+            xq, itq, _ = O.pagerank_pull(n, irow, icol, outdeg, 0.85, 1e-12, threads=threads,
+                                         personalised=True, source=src)
+            assert itq == itp
+            assert np.abs(xq - xp).max() < 1e-15 and np.abs(xq - xp).sum() < 1e-12
+        if last is not None:
+            assert np.abs(xq - last).sum() > 1e-3
+        last = xq
+
+
+@pytest.mark.parametrize("shape", ["rmat", "grid", "zero_weights"])
+def test_bucket_queue_oracles_equal_heap_oracles(shape):
+    """Dial's bucket queues give Dijkstra's and max-min Dijkstra's answers (RMAT with weights
+    1..64, a grid with shortcuts and weights 1..1000, a graph with zero-weight edges), from
+    several sources; swapping the two widest rules changes the answer."""
+    if shape == "rmat":
+        s, d, w = eg.gen_rmat(13, 16, seed=9, wmin=1, wmax=64)
+        n = 1 << 13
+    elif shape == "grid":
+        s, d, w = eg.gen_grid(60, 50, 0.01, seed=3, wmin=1, wmax=1000)
+        n = 3000
+    else:
+        s, d, w = eg.gen_rmat(11, 8, seed=4, wmin=0, wmax=3)
+        n = 1 << 11
+    row, col, wo = O.csr(n, s, d, w)
+    outdeg = np.diff(row)
+    for src in (int(np.argmax(outdeg)), 0, 5):
+        assert np.array_equal(O.dial(n, row, col, wo, src), O.dijkstra(n, row, col, wo, src))
+        assert np.array_equal(O.dial_widest(n, row, col, wo, src), O.widest(n, row, col, wo, src))
+    with pytest.raises(ValueError):
+        O.dial(n, row, col, wo, 0, wmax=int(wo.max()) - 1)
+
+
+def test_bucket_queue_oracles_equal_bruteforce():
+    with open(os.path.join(GOLD, "tiny_bruteforce.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        s, d, w = (np.array(c[k], np.uint32) for k in ("src", "dst", "w"))
+        row, col, wo = O.csr(c["n"], s, d, w)
+        assert O.dial(c["n"], row, col, wo, c["source"]).tolist() == c["sssp"]
+        assert O.dial_widest(c["n"], row, col, wo, c["source"]).tolist() == c["sswp"]
+
+
+def test_csr_build_is_a_stable_counting_sort():
+    """The parallel CSR build keeps every row in input order (equal to a stable argsort by
+    source), with hubs, empty rows and duplicate edges."""
+    s, d, w = eg.gen_rmat(14, 16, seed=12, wmin=1, wmax=64)
+    n = 1 << 14
+    row, col, wo = O.csr(n, s, d, w)
+    order = np.argsort(s, kind="stable")
+    assert np.array_equal(row, np.concatenate([[0], np.cumsum(np.bincount(s, minlength=n))]).astype(np.uint64))
+    assert np.array_equal(col, d[order]) and np.array_equal(wo, w[order])
