@@ -133,6 +133,8 @@ _sig("eeg_csr_export_derived", C.c_int, C.c_void_p, u32, C.c_void_p, C.c_void_p,
 _sig("eeg_run", C.c_int, C.c_void_p, u32, u32, P(Opts), C.c_void_p, P(Stats))
 _sig("eeg_run_device", C.c_int, C.c_void_p, u32, u32, P(Opts), C.c_void_p, C.c_void_p,
      P(Stats))
+_sig("eeg_run_device_noalloc", C.c_int, C.c_void_p, u32, u32, P(Opts), C.c_void_p, C.c_void_p,
+     P(Stats))
 _sig("eeg_run_batch", C.c_int, C.c_void_p, u32, C.c_void_p, u32, P(Opts), C.c_void_p, P(Stats))
 _sig("eeg_run_packed", C.c_int, C.c_void_p, u32, u32, C.c_void_p, P(Opts), C.c_void_p,
      P(Stats))
@@ -151,7 +153,7 @@ EXPORTS = [
     "eeg_graph_create", "eeg_graph_create_device", "eeg_graph_create_rmat", "eeg_graph_destroy",
     "eeg_graph_get_info", "eeg_graph_prepare", "eeg_tigr_export", "eeg_csr_export",
     "eeg_csr_export_derived", "eeg_run",
-    "eeg_run_device", "eeg_run_batch", "eeg_run_packed", "eeg_autotune", "eeg_server_create",
+    "eeg_run_device", "eeg_run_device_noalloc", "eeg_run_batch", "eeg_run_packed", "eeg_autotune", "eeg_server_create",
     "eeg_server_submit", "eeg_server_wait", "eeg_server_cancel", "eeg_server_destroy",
     "eeg_energy_read",
 ]

@@ -206,5 +206,18 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st, Work* wk = nullptr, hipStream_t sk = nullptr);
 
+// Device buffers of a traversal that allocates nothing (n + 1 words each, Ctl aligned).
+struct FixedState {
+    uint32_t* lab;
+    uint32_t* q[2];
+    uint32_t* s[2];
+    uint32_t* stamp;
+    Ctl* ctl;
+};
+// BFS/SSSP/SSWP/CC in sync_push_dd (compact queues) or the topology-driven sweep, on `fs`,
+// on stream `s`; the graph's tables must exist (eeg_graph_prepare).  Labels stay in fs.lab
+// in internal vertex order.
+eeg_status run_traversal_fixed(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                               const FixedState& fs, hipStream_t s, eeg_stats* st);
 
 }  // namespace eeg

@@ -1027,4 +1027,126 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     return rc;
 }
 
+// A traversal on caller-given device buffers that allocates nothing: no device or pinned
+// memory, events or executable graphs.  Frontier queues (sync_push_dd, compact) or the
+// topology-driven sweep; the host reads the control block after each chunk of launches.
+// The dedup stamps are cleared at the start, so nothing carries over between runs.
+eeg_status run_traversal_fixed(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
+                               const FixedState& fs, hipStream_t s, eeg_stats* st) {
+    const auto t_host0 = std::chrono::steady_clock::now();
+    const uint32_t K = o.K ? o.K : 8;
+    if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
+    if (!block_ok(o.block)) return fail(EEG_ERR_INVALID_ARG, "block must be 256, 512 or 1024 (got %u)", o.block);
+    const uint32_t block = o.block ? o.block : (uint32_t)WG;
+    if (o.wgs_per_cu > 64) return fail(EEG_ERR_INVALID_ARG, "wgs_per_cu must be <= 64");
+    if (o.collect_activity || o.timing)
+        return fail(EEG_ERR_UNSUPPORTED, "activity and per-launch timing need allocations");
+    int mode;
+    switch (o.variant) {
+        case EEG_ASYNC_PUSH_TD:
+        case EEG_SYNC_PUSH_TD: mode = M_TD; break;
+        case EEG_SYNC_PUSH_DD:
+            if (o.frontier == EEG_FRONTIER_FLAGS)
+                return fail(EEG_ERR_UNSUPPORTED, "flag-scan frontiers are not available without allocation");
+            mode = M_QUEUE;
+            break;
+        case EEG_ASYNC_PUSH_DD:
+            return fail(EEG_ERR_UNSUPPORTED, "async_push_dd keeps per-virtual-node state; not available without allocation");
+        default: return fail(EEG_ERR_INVALID_ARG, "unknown variant %u", o.variant);
+    }
+    const int alg = (int)algo;
+    if (alg > A_CC) return fail(EEG_ERR_UNSUPPORTED, "only BFS, SSSP, SSWP and CC run without allocation");
+    if ((alg == A_SSSP || alg == A_SSWP) && !g->weighted)
+        return fail(EEG_ERR_INVALID_ARG, "SSSP/SSWP need a weighted graph");
+    if (alg == A_CC && !g->sym.built)
+        return fail(EEG_ERR_INVALID_ARG, "CC needs the symmetrised CSR: call eeg_graph_prepare first");
+    Csr& c = (alg == A_CC) ? g->sym : g->out;
+    Tigr* tt = (alg == A_CC) ? g->tsym : g->tout;
+    if (!tt[k_index(K)].built)
+        return fail(EEG_ERR_INVALID_ARG, "Tigr tables for K = %u not built: call eeg_graph_prepare first", K);
+    const Tigr& t = tt[k_index(K)];
+    const uint32_t n = g->n;
+    const uint32_t max_iters = o.max_iters ? o.max_iters : (n + 2);
+
+    const uint32_t fill = (alg == A_SSWP) ? 0u : INF;
+    const uint32_t at_src = (alg == A_SSWP) ? INF : 0u;
+    k_init_labels<<<grid_for(n, 8192), WG, 0, s>>>(fs.lab, n, fill, at_src, source,
+                                                   alg == A_CC ? g->inv.as<uint32_t>() : nullptr);
+    if (mode == M_QUEUE) k_fill_u32<<<grid_for(n, 8192), WG, 0, s>>>(fs.stamp, n, 0u);
+    const bool seeded = (alg != A_CC);
+    k_init_ctl<<<1, 64, 0, s>>>(fs.ctl, c.rowp(), K, (seeded && mode == M_QUEUE) ? 1 : 0, source, fs.q[0], fs.s[0]);
+    EEG_HIP(hipGetLastError());
+
+    const int cus = cu_count(g->device);
+    const uint32_t TV = TILE_EDGES / K;
+    const uint64_t static_tiles = (t.P + TV - 1) / TV;
+    const uint32_t wpc = o.wgs_per_cu ? o.wgs_per_cu : std::max(1u, 24u * (uint32_t)WG / block);
+    const unsigned grid_static = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(static_tiles, (uint64_t)cus * wpc));
+    const unsigned grid_queue = (unsigned)cus * wpc;
+    KernelFn kfn = pick_kernel(alg, K, mode, block);
+    RelaxArgs ra{};
+    ra.row = c.rowp();
+    ra.col = c.colp();
+    ra.wgt = c.wp();
+    ra.vnc = t.vnc.as<uint32_t>();
+    ra.lab = fs.lab;
+    ra.stamp = fs.stamp;
+    ra.ctl = fs.ctl;
+    ra.cap_out = n;
+    ra.n_static = t.nz;
+    ra.P_static = (uint32_t)t.P;
+    ra.count_stats = o.count_work ? 1u : 0u;
+    const uint32_t base = 1;  // stamps were cleared to 0: iteration it stamps 1 + it
+    auto launch = [&](uint32_t it) -> eeg_status {
+        ra.it = it;
+        ra.stampv = base + it;
+        unsigned grid = grid_static;
+        if (mode == M_QUEUE && !(it == 1 && !seeded)) {
+            ra.qin = fs.q[(it - 1) & 1];
+            ra.sin = fs.s[(it - 1) & 1];
+            ra.qslot_in = (int)((it - 1) % 3);
+            grid = grid_queue;
+        } else {
+            ra.qin = t.vlist.as<uint32_t>();
+            ra.sin = t.vscan.as<uint32_t>();
+            ra.qslot_in = -1;
+        }
+        if (mode == M_QUEUE) {
+            ra.qout = fs.q[it & 1];
+            ra.sout = fs.s[it & 1];
+            ra.qslot_out = (int)(it % 3);
+        }
+        void* args[] = {&ra};
+        EEG_HIP(hipLaunchKernel((const void*)kfn, dim3(grid), dim3(block), args, 0, s));
+        return EEG_OK;
+    };
+    uint32_t it = 1, chunk = 2;
+    Ctl hc;
+    memset(&hc, 0, sizeof(hc));
+    eeg_status rc = EEG_OK;
+    while (true) {
+        for (uint32_t k = 0; k < chunk && it <= max_iters; ++k, ++it) EEG_TRY(launch(it));
+        EEG_HIP(hipMemcpyAsync(&hc, fs.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        EEG_HIP(hipStreamSynchronize(s));
+        if (hc.err || hc.done) break;
+        if (it > max_iters) {
+            rc = fail(EEG_ERR_NOT_CONVERGED, "no fixpoint after %u iterations", max_iters);
+            break;
+        }
+        chunk = std::min<uint32_t>(chunk * 2, 16);
+    }
+    if (rc == EEG_OK && hc.err) rc = fail(EEG_ERR_DEVICE, "frontier queue overflow (err=%u)", hc.err);
+    if (st) {
+        st->iterations = hc.iters;
+        st->launches = it - 1;
+        st->edges_relaxed = hc.stat[2];
+        st->vnodes_processed = hc.stat[1];
+        st->vertices_touched = hc.stat[0];
+        st->label_updates = hc.stat[3];
+        st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+        st->ms_traversal = st->ms_total;
+    }
+    return rc;
+}
+
 }  // namespace eeg

@@ -225,6 +225,23 @@ eeg_status eeg_run(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts*
 eeg_status eeg_run_device(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts* opts,
                           void* d_labels_out, void* stream, eeg_stats* stats);
 
+/* ---------------- single query without allocation ----------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The same query as
+ * eeg_run_device (labels identical), but it calls no hipMalloc, hipFree or other allocating
+ * routine: the traversal state lives in a fixed device array of the library (about 48 MiB per
+ * GPU, held until the process ends), so graphs are limited to EEG_NOALLOC_MAX_VERTICES
+ * vertices (larger ones are rejected with EEG_ERR_INVALID_ARG).  BFS, SSSP, SSWP and CC in
+ * sync_push_dd with compact queues or the topology-driven sweeps (async_push_dd, flag scans,
+ * activity and per-launch timing return EEG_ERR_UNSUPPORTED).  The graph's tables must exist
+ * beforehand: eeg_graph_prepare(g, K, EEG_PREP_TIGR | EEG_PREP_SYM).  All calls on one GPU
+ * share the array: calls that overlap on one GPU race, so the caller serialises them.
+ * stats->ms_traversal is host wall time (no events are created). */
+#define EEG_NOALLOC_MAX_VERTICES 2097152u
+eeg_status eeg_run_device_noalloc(eeg_graph* g, uint32_t algo, uint32_t source,
+                                  const eeg_opts* opts, void* d_labels_out, void* stream,
+                                  eeg_stats* stats);
+
 /* ---------------- batched queries ----------------
  * nsrc sources in one traversal: BFS runs multi-source BFS with 64-bit visit masks
  * (ceil(nsrc/64) words per vertex, passes of 256); SSSP/SSWP run vertex-major label planes
