@@ -320,6 +320,24 @@ def batched_section(eg, a):
                       "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
         log(f"batched {alg}: {multi[alg]}")
     res[f"rmat{a.batch_scale}_256_sources"] = multi
+    # personalised PageRank: 64 restart vertices (the first 64 sources) as planes of 32
+    psrc = srcs[:64]
+    g.run("ppr", int(psrc[0]), want_labels=False)
+    g.run_batch("ppr", psrc[:32], want_labels=False)
+    lat = []
+    t = time.perf_counter()
+    for x in psrc:
+        g.run("ppr", int(x), want_labels=False)
+        lat.append((time.perf_counter() - t) * 1e3)
+    one = time.perf_counter() - t
+    t = time.perf_counter()
+    _, st = g.run_batch("ppr", psrc, want_labels=False)
+    bat = time.perf_counter() - t
+    res[f"rmat{a.batch_scale}_64_ppr_sources"] = {
+        "one_at_a_time_qps": round(64 / one, 1), "batched_qps": round(64 / bat, 1), "speedup": round(one / bat, 2),
+        "one_at_a_time_p99_ms": pct(lat, 99), "batched_p99_ms": round(bat * 1e3, 3),
+        "iterations": st["iterations"], "plane_iterations": st["launches"]}
+    log(f"batched ppr: {res[f'rmat{a.batch_scale}_64_ppr_sources']}")
     g.close()
 
     rng = np.random.default_rng(5)
@@ -408,10 +426,11 @@ def grid_section(eg, a):
         r = {}
         for name, var, fr in (("sync_compact", "sync_push_dd", "compact"), ("sync_flags", "sync_push_dd", "flags"),
                               ("async_push_dd", "async_push_dd", "compact"), ("async_push_td", "async_push_td", "compact")):
-            g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False)
+            c = g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False, count_work=True)[1]
             _, st = g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False)
             r[name] = {"ms": round(st["ms_traversal"], 3), "gteps": round(m / st["ms_traversal"] / 1e6, 2),
-                       "iterations": st["iterations"]}
+                       "iterations": st["iterations"], "alg_bytes": c["bytes_alg"],
+                       "hbm_frac": round(c["bytes_alg"] / (st["ms_traversal"] * 1e-3) / 8e12, 4)}
         res[alg] = r
         log(f"grid {alg}: {r}")
     g.close()
@@ -587,6 +606,7 @@ def main():
         key = f"{alg}/{var}"
         per_alg[key] = {"gteps": m / ms / 1e6, "ms": round(ms, 4), "iterations": sts[0]["iterations"],
                         "edges_relaxed": c["edges_relaxed"], "label_updates": c["label_updates"],
+                        "alg_bytes": c["bytes_alg"],
                         "kernel_ms_avg": round(avg_launch_ms, 5),
                         "alg_gbs": bytes_per_launch / avg_launch_ms / 1e6 if kn else None}
         if alg == "pr":
