@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--serve-qps", default="25,50,100,200,400,1000,5000",
                    help="offered loads (queries/s) of the serving sweep")
     p.add_argument("--serve-queries", type=int, default=200, help="queries per serving point")
+    p.add_argument("--serve-queries-top", type=int, default=600,
+                   help="queries at the highest offered load (the saturated throughput point)")
     p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp"),
                    help="only run two multi-source passes (the second timed) of this algorithm on the "
                         "batched section's graph and sources, for profiling the batched kernels")
@@ -362,6 +364,7 @@ def batched_section(eg, a):
                        "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
         log(f"packed {alg}: {packed[alg]}")
     res["packed_64_rmat_graphs"] = packed
+    res["packed_64_rmat_graphs_tuned"] = packed_tuned_section(eg, gs, psrc)
     for x in gs:
         x.close()
     return res
@@ -389,6 +392,41 @@ def single_query_section(g, K):
                                    "iterations": c["iterations"], "alg_bytes": c["bytes_alg"],
                                    "alg_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
             log(f"single query rmat24 {alg}/{var}: {out[f'{alg}/{var}']}")
+    return out
+
+
+def packed_tuned_section(eg, gs, psrc):
+    """The packed workload with tuned options on both sides: every graph's one-at-a-time query
+    runs with eeg_autotune's choice for that graph (choices cached per graph shape), the packed
+    pass with the fastest of the variant / frontier / workgroup-size configurations tried on
+    it.  Both sides timed as in the default comparison."""
+    import eegraph_amd as eg_
+
+    out = {}
+    forms = [dict(variant="sync_push_dd"), dict(variant="sync_push_dd", frontier="flags"),
+             dict(variant="async_push_dd"), dict(variant="async_push_td")]
+    for alg in ("bfs", "sssp", "sswp", "cc"):
+        tuned = [x.autotune(alg, [int(y)])[0] for x, y in zip(gs, psrc)]
+        for x, y, o in zip(gs, psrc, tuned):
+            x.run(alg, int(y), opts=o, want_labels=False)
+        t = time.perf_counter()
+        for x, y, o in zip(gs, psrc, tuned):
+            x.run(alg, int(y), opts=o, want_labels=False)
+        one = time.perf_counter() - t
+        best = None
+        for f in forms:
+            for block in (256, 512):
+                o = eg_.make_opts(K=8, **f)
+                o.block = block
+                eg.run_packed(gs, alg, psrc, opts=o, want_labels=False)
+                t = time.perf_counter()
+                eg.run_packed(gs, alg, psrc, opts=o, want_labels=False)
+                dt = time.perf_counter() - t
+                if best is None or dt < best[0]:
+                    best = (dt, dict(f, block=block))
+        out[alg] = {"one_at_a_time_qps": round(64 / one, 1), "batched_qps": round(64 / best[0], 1),
+                    "speedup": round(one / best[0], 2), "packed_options": best[1]}
+        log(f"packed tuned {alg}: {out[alg]}")
     return out
 
 
@@ -454,10 +492,10 @@ def serving_section(eg, a):
     g.prepare(8, 1 | 4)  # Tigr of the out- and in-edge CSRs (PPR gathers over in-edges)
     log(f"serving graph n={n} m={g.info()['m']} built in {time.perf_counter() - t:.2f} s")
     rng = np.random.default_rng(6)
-    nq = a.serve_queries
-    mix = rng.choice(["bfs", "sssp", "sswp", "ppr"], size=nq, p=[0.4, 0.3, 0.2, 0.1])
-    srcs = rng.integers(0, n, nq)
-    gaps = rng.exponential(1.0, nq)  # unit-rate gaps, scaled per offered load
+    nmax = max(a.serve_queries, a.serve_queries_top)
+    mix = rng.choice(["bfs", "sssp", "sswp", "ppr"], size=nmax, p=[0.4, 0.3, 0.2, 0.1])
+    srcs = rng.integers(0, n, nmax)
+    gaps = rng.exponential(1.0, nmax)  # unit-rate gaps, scaled per offered load
 
     # one at a time: service time of every query (eeg_run each, serially)
     o = eg.make_opts(variant="sync_push_dd", K=8)
@@ -466,21 +504,23 @@ def serving_section(eg, a):
     meter = EnergyMeter(eg)
     svc = []
     t1 = time.perf_counter()
-    for i in range(nq):
+    for i in range(nmax):
         t = time.perf_counter()
         g.run(mix[i], int(srcs[i]), opts=o, want_labels=False)
         svc.append(time.perf_counter() - t)
     t2 = time.perf_counter()
     p1 = meter.power(t1, t2)
-    one_j = round(p1[0] * (t2 - t1) / nq, 4) if p1 else None
-    log(f"one at a time: {nq} queries in {t2 - t1:.2f} s ({nq / (t2 - t1):.1f} q/s)")
+    one_j = round(p1[0] * (t2 - t1) / nmax, 4) if p1 else None
+    log(f"one at a time: {nmax} queries in {t2 - t1:.2f} s ({nmax / (t2 - t1):.1f} q/s)")
 
     srv = eg.Server(g, discard_labels=True)
     for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
         srv.wait(srv.submit(alg, 0))
     points = []
-    for rate in [float(x) for x in a.serve_qps.split(",")]:
-        arrive = np.cumsum(gaps) / rate
+    rates = [float(x) for x in a.serve_qps.split(",")]
+    for rate in rates:
+        nq = a.serve_queries_top if rate == rates[-1] else a.serve_queries
+        arrive = np.cumsum(gaps[:nq]) / rate
         tickets = [None] * nq
 
         def producer():
@@ -515,7 +555,7 @@ def serving_section(eg, a):
             fin = max(arrive[i], fin) + svc[i]
             lat1.append((fin - arrive[i]) * 1e3)
         one_s = fin - arrive[0]
-        pt = {"offered_qps": rate,
+        pt = {"offered_qps": rate, "queries": nq,
               "served": {"qps": round(nq / served_s, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                          "per_algorithm_p50_p99_ms": {k: [pct(v, 50), pct(v, 99)] for k, v in per.items()},
                          "j_per_query": round(ps[0] * (tw1 - tw0) / nq, 4) if ps else None},
@@ -526,10 +566,11 @@ def serving_section(eg, a):
     srv.close()
     meter.close()
     g.close()
-    out = {"workload": f"rmat{a.serve_scale}_ef16_seed6, {nq} queries per point, Poisson arrivals, "
+    out = {"workload": f"rmat{a.serve_scale}_ef16_seed6, {a.serve_queries} queries per point "
+                       f"({a.serve_queries_top} at the highest load), Poisson arrivals, "
                        f"40/30/20/10 bfs/sssp/sswp/ppr",
-           "queries": nq,
-           "one_at_a_time_capacity_qps": round(nq / sum(svc), 1),
+           "queries": a.serve_queries,
+           "one_at_a_time_capacity_qps": round(nmax / sum(svc), 1),
            "one_at_a_time_j_per_query": one_j,
            "points": points,
            "qps_ratio": points[-1]["qps_ratio"] if points else None}

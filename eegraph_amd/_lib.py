@@ -88,7 +88,7 @@ class ServerCfg(C.Structure):
     _fields_ = [("label_budget_bytes", u64), ("max_batch", u32), ("K", u32),
                 ("damping", C.c_float), ("ppr_epsilon", C.c_double), ("flags", u32),
                 ("lane_wgs_per_cu", u32), ("lanes", u32), ("first_iteration", u32),
-                ("reserved", u32 * 4)]
+                ("ppr_columns", u32), ("reserved", u32 * 3)]
 
 
 SERVER_DISCARD_LABELS, SERVER_PLANE_ENGINES, SERVER_NO_BFS_ENGINE, SERVER_NO_PPR_ENGINE = 1, 2, 4, 8

@@ -259,7 +259,8 @@ class Server:
 
     def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
                  ppr_epsilon=0.0, discard_labels=False, lane_wgs_per_cu=0, lanes=0,
-                 plane_engines=False, bfs_engine=True, first_iteration=0, ppr_engine=True):
+                 plane_engines=False, bfs_engine=True, first_iteration=0, ppr_engine=True,
+                 ppr_columns=0):
         cfg = L.ServerCfg()
         cfg.label_budget_bytes = label_budget_bytes
         cfg.max_batch = max_batch
@@ -273,6 +274,7 @@ class Server:
         cfg.lane_wgs_per_cu = lane_wgs_per_cu
         cfg.lanes = lanes
         cfg.first_iteration = first_iteration
+        cfg.ppr_columns = ppr_columns
         self.graph = graph  # keeps the graph alive while the server runs
         self.n = graph.info()["n"]
         self.discard = discard_labels

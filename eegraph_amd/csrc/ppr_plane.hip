@@ -106,10 +106,8 @@ struct PlaneArgsP {
     uint32_t nseg, nmulti;
     const uint32_t* outdeg;
     const float* c_old;
-    const float* xh_old;
-    const float* xl_old;
-    float* xh_new;
-    float* xl_new;
+    const float2* x_old;     // ranks as compensated pairs (hi, lo)
+    float2* x_new;
     float* c_new;
     float2* spill;
     double* res_part;
@@ -117,19 +115,22 @@ struct PlaneArgsP {
     PprCols cols;
 };
 
+// Old rank of entry vertex v in column q (x0 = e_src for a fresh column, nothing read).
+template <int B>
+__device__ __forceinline__ float2 old_rank(const PlaneArgsP& a, uint32_t v, uint32_t q, uint32_t src, bool fresh) {
+    return fresh ? make_float2(v == src ? 1.0f : 0.0f, 0.0f) : a.x_old[(uint64_t)v * B + q];
+}
+
 // New rank of entry vertex v in column q from its gathered sum; adds the L1 change to res.
 template <int B>
-__device__ __forceinline__ void finish_col(const PlaneArgsP& a, uint32_t v, uint32_t q, uint32_t src, bool fresh,
-                                           Comp sum, double& res) {
+__device__ __forceinline__ void finish_col(const PlaneArgsP& a, uint32_t v, uint32_t q, uint32_t src, float2 xo,
+                                           uint32_t od, Comp sum, double& res) {
     float hi, lo;
     rank_pair(v == src ? 1.0f - a.damping : 0.0f, a.damping, sum, &hi, &lo);
     const uint64_t o = (uint64_t)v * B + q;
-    a.xh_new[o] = hi;
-    a.xl_new[o] = lo;
-    const uint32_t od = a.outdeg[v];
+    a.x_new[o] = make_float2(hi, lo);
     a.c_new[o] = od ? (hi + lo) / (float)od : 0.0f;
-    const double old = fresh ? (v == src ? 1.0 : 0.0) : (double)a.xh_old[o] + (double)a.xl_old[o];
-    res += fabs(((double)hi + (double)lo) - old);
+    res += fabs(((double)hi + (double)lo) - ((double)xo.x + (double)xo.y));
 }
 
 // Per-column FP64 residual of a workgroup: lanes q, q+B, ... of every wave, in a fixed order.
@@ -164,6 +165,14 @@ __global__ __launch_bounds__(WG) void k_pps_pull(PlaneArgsP a) {
         if (t >= a.nseg) continue;
         const uint4 dsc = a.sdesc[t];
         const uint32_t e0 = dsc.x, len = dsc.y & 0x7fffffffu, v = dsc.z;
+        const bool single = (dsc.y >> 31) != 0;
+        // what finishing the vertex reads is loaded before the gathers, so its latency overlaps
+        float2 xo = make_float2(0.0f, 0.0f);
+        uint32_t od = 0;
+        if (single) {
+            xo = old_rank<B>(a, v, q, src, fresh);
+            od = a.outdeg[v];
+        }
         Comp acc{0.0f, 0.0f};
         // source indices of the next 8 edges are loaded while the current 8 rows are gathered
         uint32_t u[8];
@@ -182,7 +191,7 @@ __global__ __launch_bounds__(WG) void k_pps_pull(PlaneArgsP a) {
             for (int k = 0; k < 8; ++k)
                 if (b + k < len) acc = comp_add(acc, Comp{cv[k], 0.0f});
         }
-        if (dsc.y >> 31) finish_col<B>(a, v, q, src, fresh, acc, res);
+        if (single) finish_col<B>(a, v, q, src, xo, od, acc, res);
         else a.spill[(uint64_t)dsc.w * B + q] = make_float2(acc.s, acc.e);
     }
     store_residual<B>(res, a.res_part);
@@ -202,12 +211,15 @@ __global__ __launch_bounds__(WG) void k_pps_fix(PlaneArgsP a, double* res_part) 
         if (j >= a.nmulti) continue;
         const uint32_t i = a.mlist[j];
         const uint32_t ns = a.sscan[i + 1] - a.sscan[i], mb = a.mbase[i];
+        const uint32_t v = a.ent[i];
+        const float2 xo = old_rank<B>(a, v, q, src, fresh);
+        const uint32_t od = a.outdeg[v];
         Comp acc{0.0f, 0.0f};
         for (uint32_t k = 0; k < ns; ++k) {
             const float2 p = a.spill[(uint64_t)(mb + k) * B + q];
             acc = comp_add(acc, Comp{p.x, p.y});
         }
-        finish_col<B>(a, a.ent[i], q, src, fresh, acc, res);
+        finish_col<B>(a, v, q, src, xo, od, acc, res);
     }
     store_residual<B>(res, res_part);
 }
@@ -234,15 +246,14 @@ __global__ __launch_bounds__(WG) void k_pps_reduce(const double* __restrict__ pa
 // out[u] = rank of original vertex u in column q: the plane value for entries, the restart
 // term alone for vertices without edges.
 template <int B>
-__global__ void k_pps_extract(const float* __restrict__ xh, const float* __restrict__ xl,
-                              const uint64_t* __restrict__ in_row, const uint32_t* __restrict__ outdeg,
-                              const uint32_t* __restrict__ perm, uint32_t n, uint32_t q, uint32_t src,
-                              float restart, float* __restrict__ out) {
+__global__ void k_pps_extract(const float2* __restrict__ x, const uint64_t* __restrict__ in_row,
+                              const uint32_t* __restrict__ outdeg, const uint32_t* __restrict__ perm, uint32_t n,
+                              uint32_t q, uint32_t src, float restart, float* __restrict__ out) {
     for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t v = perm[u];
         const bool entry = in_row[v + 1] > in_row[v] || outdeg[v] > 0;
-        const uint64_t o = (uint64_t)v * B + q;
-        out[u] = entry ? xh[o] + xl[o] : (v == src ? restart : 0.0f);
+        const float2 r = x[(uint64_t)v * B + q];
+        out[u] = entry ? r.x + r.y : (v == src ? restart : 0.0f);
     }
 }
 
@@ -313,9 +324,8 @@ eeg_status PprPlane::init(eeg_graph* g, uint32_t Bc, float d, hipStream_t s) {
     const size_t plane = (size_t)n * B * 4 + 16;
     for (int k = 0; k < 2; ++k) {
         EEG_TRY(c[k].alloc(plane));
-        EEG_TRY(xh[k].alloc(plane));
-        EEG_TRY(xl[k].alloc(plane));
-        for (DevMem* m : {&c[k], &xh[k], &xl[k]}) EEG_HIP(hipMemsetAsync(m->p, 0, m->bytes, s));
+        EEG_TRY(x[k].alloc(2 * plane));
+        for (DevMem* m : {&c[k], &x[k]}) EEG_HIP(hipMemsetAsync(m->p, 0, m->bytes, s));
     }
     EEG_TRY(spill.alloc((size_t)t.nspill * B * 8 + 16));
     int cus = 256;
@@ -343,11 +353,9 @@ eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipS
     a.nmulti = t.nmulti;
     a.outdeg = g->outdeg.as<uint32_t>();
     a.c_old = c[rb].as<float>();
-    a.xh_old = xh[rb].as<float>();
-    a.xl_old = xl[rb].as<float>();
+    a.x_old = x[rb].as<float2>();
     a.c_new = c[wb].as<float>();
-    a.xh_new = xh[wb].as<float>();
-    a.xl_new = xl[wb].as<float>();
+    a.x_new = x[wb].as<float2>();
     a.spill = spill.as<float2>();
     a.res_part = res_part.as<double>();
     a.damping = damping;
@@ -370,20 +378,18 @@ eeg_status PprPlane::extract(eeg_graph* g, uint32_t j, uint32_t q, uint32_t src,
     const int wb = (int)(j & 1u);
     const unsigned gr = grid_of(n, 8192);
     if (B == 16)
-        k_pps_extract<16><<<gr, WG, 0, s>>>(xh[wb].as<float>(), xl[wb].as<float>(), g->in.rowp(),
-                                            g->outdeg.as<uint32_t>(), g->perm.as<uint32_t>(), n, q, src,
-                                            1.0f - damping, out);
+        k_pps_extract<16><<<gr, WG, 0, s>>>(x[wb].as<float2>(), g->in.rowp(), g->outdeg.as<uint32_t>(),
+                                            g->perm.as<uint32_t>(), n, q, src, 1.0f - damping, out);
     else
-        k_pps_extract<32><<<gr, WG, 0, s>>>(xh[wb].as<float>(), xl[wb].as<float>(), g->in.rowp(),
-                                            g->outdeg.as<uint32_t>(), g->perm.as<uint32_t>(), n, q, src,
-                                            1.0f - damping, out);
+        k_pps_extract<32><<<gr, WG, 0, s>>>(x[wb].as<float2>(), g->in.rowp(), g->outdeg.as<uint32_t>(),
+                                            g->perm.as<uint32_t>(), n, q, src, 1.0f - damping, out);
     EEG_HIP(hipGetLastError());
     return EEG_OK;
 }
 
-// Batched PPR (eeg_run_batch): sources in passes of B columns; every column iterates from
-// e_src until its own L1 change drops below epsilon and is then read out, so each answer is
-// the single-query iteration's, summed in a fixed order.
+// Batched PPR (eeg_run_batch): up to B columns in flight; every column iterates from e_src
+// until its own L1 change drops below epsilon, is read out and takes the next source, so each
+// answer is the single-query iteration's, summed in a fixed order.
 eeg_status run_ppr_batch(eeg_graph* g, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o, float* out_host,
                          eeg_stats* st) {
     const float d = o.damping > 0 ? o.damping : 0.85f;
@@ -406,39 +412,44 @@ eeg_status run_ppr_batch(eeg_graph* g, const uint32_t* sources, uint32_t nsrc, c
     EEG_HIP(hipEventRecord(e0, s));
     uint32_t total_iters = 0, max_col_iters = 0;
     eeg_status rc = EEG_OK;
-    for (uint32_t b = 0; b < nsrc && rc == EEG_OK; b += B) {
-        const uint32_t cnt = std::min(B, nsrc - b);
-        PprCols cols;
-        for (uint32_t q = 0; q < 32; ++q) cols.src[q] = PPR_IDLE;
-        cols.fresh = 0;
-        for (uint32_t q = 0; q < cnt; ++q) {
-            cols.src[q] = g->h_perm[sources[b + q]];
+    // columns are refilled as soon as their query converges (the next source starts fresh in
+    // the following iteration), so a pass never waits for its slowest column
+    PprCols cols;
+    for (uint32_t q = 0; q < 32; ++q) cols.src[q] = PPR_IDLE;
+    cols.fresh = 0;
+    std::vector<uint32_t> colq(B, 0), start(B, 0);
+    uint32_t next = 0, left = nsrc;
+    for (uint32_t j = 1; left > 0 && rc == EEG_OK; ++j) {
+        for (uint32_t q = 0; q < B && next < nsrc; ++q) {
+            if (cols.src[q] != PPR_IDLE) continue;
+            cols.src[q] = g->h_perm[sources[next]];
             cols.fresh |= 1u << q;
+            colq[q] = next++;
+            start[q] = j;
         }
-        std::vector<uint32_t> it_of(cnt, 0);
-        uint32_t left = cnt;
-        for (uint32_t j = 1; left > 0; ++j) {
-            if (j > max_iters) {
-                rc = fail(EEG_ERR_NOT_CONVERGED, "PPR residual above %g after %u iterations", eps, max_iters);
-                break;
-            }
-            EEG_TRY(pl.iterate(g, j, cols, s));
-            EEG_HIP(hipMemcpyAsync(hres.data(), pl.resid.p, (size_t)B * 8, hipMemcpyDeviceToHost, s));
-            EEG_HIP(hipStreamSynchronize(s));
-            ++total_iters;
-            cols.fresh = 0;
-            for (uint32_t q = 0; q < cnt; ++q) {
-                if (cols.src[q] == PPR_IDLE || !(hres[q] < eps)) continue;
-                it_of[q] = j;
-                if (out_host) {
-                    EEG_TRY(pl.extract(g, j, q, cols.src[q], dout.as<float>(), s));
-                    EEG_HIP(hipMemcpyAsync(out_host + (uint64_t)(b + q) * n, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        EEG_TRY(pl.iterate(g, j, cols, s));
+        EEG_HIP(hipMemcpyAsync(hres.data(), pl.resid.p, (size_t)B * 8, hipMemcpyDeviceToHost, s));
+        EEG_HIP(hipStreamSynchronize(s));
+        ++total_iters;
+        cols.fresh = 0;
+        for (uint32_t q = 0; q < B; ++q) {
+            if (cols.src[q] == PPR_IDLE) continue;
+            const uint32_t its = j - start[q] + 1;
+            if (!(hres[q] < eps)) {
+                if (its >= max_iters) {
+                    rc = fail(EEG_ERR_NOT_CONVERGED, "PPR residual above %g after %u iterations", eps, max_iters);
+                    break;
                 }
-                cols.src[q] = PPR_IDLE;
-                --left;
+                continue;
             }
+            max_col_iters = std::max(max_col_iters, its);
+            if (out_host) {
+                EEG_TRY(pl.extract(g, j, q, cols.src[q], dout.as<float>(), s));
+                EEG_HIP(hipMemcpyAsync(out_host + (uint64_t)colq[q] * n, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            }
+            cols.src[q] = PPR_IDLE;
+            --left;
         }
-        for (uint32_t q = 0; q < cnt; ++q) max_col_iters = std::max(max_col_iters, it_of[q]);
     }
     EEG_HIP(hipEventRecord(e1, s));
     EEG_HIP(hipStreamSynchronize(s));

@@ -26,7 +26,8 @@ eeg_status ensure_ppr_seg(eeg_graph* g);
 struct PprPlane {
     uint32_t B = 0, n = 0;
     float damping = 0.85f;
-    DevMem c[2], xh[2], xl[2];  // n x B each
+    DevMem c[2];                // n x B contributions x/outdeg
+    DevMem x[2];                // n x B ranks as compensated pairs (hi, lo)
     DevMem spill;               // nspill x B float2
     DevMem res_part;            // (grid_pull + grid_fix) x B doubles
     DevMem resid;               // B doubles: L1 change of each column in the last iteration

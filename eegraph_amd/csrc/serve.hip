@@ -893,7 +893,10 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         if (c.first_iteration) e->it = c.first_iteration;
         EEG_TRY(e->init(g));
     }
-    // PPR engine: rank planes of 32 columns, or 16 when 32 do not fit a third of the free memory
+    // PPR engine: rank planes of cfg.ppr_columns (16 or 32, default 32) columns, 16 when 32 do
+    // not fit a third of the free memory
+    if (c.ppr_columns != 0 && c.ppr_columns != 16 && c.ppr_columns != 32)
+        return fail(EEG_ERR_INVALID_ARG, "ppr_columns must be 16 or 32");
     if ((c.flags & EEG_SERVER_NO_PPR_ENGINE) == 0 && n > 0) {
         {
             std::lock_guard<std::mutex> lk(g->mu);
@@ -903,7 +906,8 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         size_t fr = 0, tot = 0;
         EEG_HIP(hipMemGetInfo(&fr, &tot));
         const uint32_t nsp = g->pprseg.nspill;
-        const uint32_t B = PprPlane::bytes_for(n, 32, nsp) <= fr / 3 ? 32u
+        const uint32_t want = c.ppr_columns ? c.ppr_columns : 32u;
+        const uint32_t B = (want == 32 && PprPlane::bytes_for(n, 32, nsp) <= fr / 3) ? 32u
                            : PprPlane::bytes_for(n, 16, nsp) <= fr / 3 ? 16u : 0u;
         if (B) {
             PprEngine* e = new (std::nothrow) PprEngine();

@@ -292,7 +292,10 @@ typedef struct eeg_server_cfg {
     uint32_t first_iteration;    /* testing: the traversal engines' iteration counter at start (0 = 1);
                                     from 0xf0000000 on an engine admits nothing until it drains,
                                     then resets its columns and restarts the counter at 1 */
-    uint32_t reserved[4];
+    uint32_t ppr_columns;        /* PPR engine width: 16 or 32 restart vertices per gather pass
+                                    (0 = 32; a pass of 16 costs ~0.6x one of 32, for lower
+                                    latency when few PPR queries are in flight) */
+    uint32_t reserved[3];
 } eeg_server_cfg;
 /* Answers are not kept: columns are given back as soon as a query converges and wait() reports
  * status and times only (labels_out must be NULL).  For throughput runs. */

@@ -176,7 +176,8 @@ def test_engine_counter_restarts_near_its_limit(rmat12):
                 _check(g, alg, int(x), lab, n, row, col, singles)
 
 
-def test_ppr_engine_columns_join_and_leave(rmat12):
+@pytest.mark.parametrize("columns", [16, 32])
+def test_ppr_engine_columns_join_and_leave(rmat12, columns):
     """More PPR queries than the engine has columns, submitted while others run: each answer
     within L1 1e-6 of its single-query run and of the FP64 oracle; queries share iterations."""
     n, s, d, w, g, row, col = rmat12
@@ -184,7 +185,7 @@ def test_ppr_engine_columns_join_and_leave(rmat12):
     srcs = [int(x) for x in rng.integers(0, n, size=70)]
     srcs[::9] = [g.info()["argmax_out_degree"]] * len(srcs[::9])
     singles = {}
-    with eg.Server(g) as srv:
+    with eg.Server(g, ppr_columns=columns) as srv:
         tk = []
         for i, x in enumerate(srcs):
             tk.append(srv.submit("ppr", x))
