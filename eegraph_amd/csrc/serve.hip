@@ -196,6 +196,7 @@ struct eeg_server {
     // single-query lanes
     std::deque<eeg::TicketP> lane_q;
     std::condition_variable lane_cv;
+    uint32_t lanes_busy = 0;  // lanes running a query (guarded by mu)
     std::vector<std::unique_ptr<eeg::Lane>> lanes;
     std::mutex prep_mu;  // builds of the derived tables a lane needs
     // answer extraction (labels of a finished column to the caller)
@@ -789,6 +790,7 @@ void lane_loop(eeg_server* srv, Lane* ln) {
             tk->state = T_RUNNING;
             tk->t_start = srv->now_ms();
             tk->batch = 1;
+            ++srv->lanes_busy;
         }
         eeg_opts o;
         memset(&o, 0, sizeof(o));
@@ -816,6 +818,7 @@ void lane_loop(eeg_server* srv, Lane* ln) {
                 rc = fail(EEG_ERR_DEVICE, "copying the answer failed");
         }
         std::lock_guard<std::mutex> lk(srv->mu);
+        --srv->lanes_busy;
         tk->status = rc;
         if (rc != EEG_OK) tk->err = eeg_last_error();
         tk->state = T_DONE;
@@ -955,7 +958,16 @@ extern "C" eeg_status eeg_server_submit(eeg_server* s, uint32_t algo, uint32_t s
                   : algo == EEG_SSWP ? E_SSWP
                   : algo == EEG_PPR  ? E_PPR
                                      : -1;
-    if (e >= 0 && s->use_eng[e]) {
+    bool to_engine = e >= 0 && s->use_eng[e];
+    if (to_engine && e == E_PPR) {
+        // A pass over all PPR columns costs several single-query passes, so a lone PPR query is
+        // answered sooner by a free lane; once lanes are busy or the engine already runs PPR
+        // queries, new ones share its passes.
+        EngineBase* pe = s->eng[E_PPR].get();
+        if (pe && pe->running == 0 && pe->pending.empty() && s->lane_q.empty() && s->lanes_busy < s->lanes.size())
+            to_engine = false;
+    }
+    if (to_engine) {
         EngineBase* eng = s->eng[e].get();
         if (!eng) return fail(EEG_ERR_REJECTED, "label budget holds no column for this algorithm");
         if (eng->broken) return fail(EEG_ERR_DEVICE, "serving engine stopped after a device error");
