@@ -320,6 +320,8 @@ def batched_section(eg, a):
         multi[alg] = {"one_at_a_time_qps": round(256 / one, 1), "batched_qps": round(256 / bat, 1),
                       "speedup": round(one / bat, 2), "one_at_a_time_p99_ms": pct(lat, 99),
                       "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
+        if alg != "bfs":
+            multi[alg]["label_bits"] = st["label_bits"]
         log(f"batched {alg}: {multi[alg]}")
     res[f"rmat{a.batch_scale}_256_sources"] = multi
     # personalised PageRank: 64 restart vertices (the first 64 sources) as planes of 32
@@ -444,7 +446,8 @@ def batched_only(eg, a):
     bat = time.perf_counter() - t
     print(json.dumps({"batched_only": a.batched_only, "workload": f"rmat{a.batch_scale}_ef16_seed4_256_sources",
                       "ms": round(bat * 1e3, 3), "ms_traversal": round(st["ms_traversal"], 3),
-                      "queries_per_s": round(256 / bat, 1), "iterations": st["iterations"]}), flush=True)
+                      "queries_per_s": round(256 / bat, 1), "iterations": st["iterations"],
+                      "label_bits": st["label_bits"]}), flush=True)
     g.close()
 
 

@@ -52,7 +52,7 @@ class Opts(C.Structure):
                 ("max_iters", u32), ("collect_activity", u32), ("timing", u32),
                 ("count_work", u32), ("damping", C.c_float), ("acc", C.c_float),
                 ("epsilon", C.c_double), ("wgs_per_cu", u32), ("energy", u32),
-                ("reserved", u32 * 6)]
+                ("label_bits", u32), ("reserved", u32 * 5)]
 
 
 class Stats(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("vnodes_processed", u64), ("vertices_touched", u64), ("label_updates", u64),
                 ("bytes_alg", C.c_double), ("pr_residual", C.c_double),
                 ("energy_j", C.c_double), ("activity", P(u32)), ("activity_cap", u32),
-                ("reserved", u32 * 7)]
+                ("label_bits", u32), ("reserved", u32 * 6)]
 
     def to_dict(self):
         d = {}

@@ -78,11 +78,11 @@ def save_edgelist(path, n, src, dst, w=None):
 
 def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timing=False,
               count_work=False, collect_activity=False, damping=0.85, epsilon=0.0, acc=0.0,
-              wgs_per_cu=0, energy=False):
+              wgs_per_cu=0, energy=False, block=0, label_bits=0):
     o = L.Opts()
     o.variant = L.VARIANTS[variant] if isinstance(variant, str) else int(variant)
     o.K = K
-    o.block = 0
+    o.block = block
     o.frontier = {"compact": L.FRONTIER_COMPACT, "flags": L.FRONTIER_FLAGS}[frontier] \
         if isinstance(frontier, str) else int(frontier)
     o.max_iters = max_iters
@@ -94,6 +94,7 @@ def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timi
     o.acc = acc
     o.wgs_per_cu = wgs_per_cu
     o.energy = 1 if energy else 0
+    o.label_bits = label_bits
     return o
 
 

@@ -329,14 +329,88 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
 }
 
 // ---------------------------------------------------------------- batched SSSP / SSWP
-// Dense variant: one wavefront per virtual node, every pending 64-query word handled with
-// lane = query, so the label reads and atomics of a wavefront are contiguous 256-byte rows.
-// Best when many queries are in flight together; the two-pass kernel below is better when
-// few are.
-template <int K>
-// 72 VGPRs at K <= 8 (7 waves per SIMD instead of 6 at 77, no spills): the row reads are
-// latency-bound, so the extra resident wave is worth more (-7 to -8% per pass on RMAT-24)
-__global__ __launch_bounds__(WG, K <= 8 ? 7 : 1) void k_plane_push_wave(PlaneArgs args) {
+// Label widths of the planes (LM): 32-bit labels (LM_U32, lane = query), or narrow labels
+// packed into 32-bit lane words (four 8-bit or two 16-bit labels), so one wavefront covers a
+// vertex's 256 queries in one or two row passes:
+//   LM_W8   SSWP with every weight <= 254: u8 labels, 255 = INF (the source), 0 = unreached.
+//           A widest-path label is a weight, INF or 0, so nothing is lost.
+//   LM_D16  SSSP: u16 distances, 0xffff = INF.  A finite candidate >= 0xffff sets Ctl::err
+//           bit 64 and is dropped; labels below it are exact (a vertex's shortest path only
+//           passes smaller distances), and the caller reruns the pass on 32-bit planes.
+// Improvements of a lane word are compare-and-swap loops: labels only improve, so every
+// failed swap means another improvement landed and the loop ends.
+enum LabelMode : int { LM_U32 = 0, LM_W8 = 1, LM_D16 = 2 };
+constexpr uint32_t PLANE_ERR_NARROW = 64u;
+
+template <int LM>
+struct Packed;
+template <>
+struct Packed<LM_W8> {
+    static constexpr uint32_t QPL = 4, BITS = 8, FM = 0xffu;  // labels per 32-bit lane word
+    __device__ static uint32_t f(uint32_t x, int i) { return (x >> (BITS * i)) & FM; }
+    // candidates of the queries in `nib` over an edge of weight w (<= 254): min(label, w)
+    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, bool&) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i)
+            if ((nib >> i) & 1u) c |= min(f(lv, i), w) << (BITS * i);
+        return c;
+    }
+    __device__ static uint32_t merge(uint32_t cur, uint32_t c) {  // per-label max
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) r |= max(f(cur, i), f(c, i)) << (BITS * i);
+        return r;
+    }
+    __device__ static uint32_t improved(uint32_t cur, uint32_t c) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) m |= (f(c, i) > f(cur, i) ? 1u : 0u) << i;
+        return m;
+    }
+};
+template <>
+struct Packed<LM_D16> {
+    static constexpr uint32_t QPL = 2, BITS = 16, FM = 0xffffu;
+    __device__ static uint32_t f(uint32_t x, int i) { return (x >> (BITS * i)) & FM; }
+    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, bool& ovf) {
+        uint32_t c = 0xffffffffu;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) {
+            const uint32_t l = f(lv, i);
+            if (!((nib >> i) & 1u) || l == FM) continue;
+            const uint64_t sm = (uint64_t)l + w;
+            if (sm < FM) c = (c & ~(FM << (BITS * i))) | ((uint32_t)sm << (BITS * i));
+            else if (sm < INF) ovf = true;  // finite, but not representable
+        }
+        return c;
+    }
+    __device__ static uint32_t merge(uint32_t cur, uint32_t c) {  // per-label min
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) r |= min(f(cur, i), f(c, i)) << (BITS * i);
+        return r;
+    }
+    __device__ static uint32_t improved(uint32_t cur, uint32_t c) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) m |= (f(c, i) < f(cur, i) ? 1u : 0u) << i;
+        return m;
+    }
+};
+
+// Dense variant: one wavefront per virtual node.  LM_U32: every pending 64-query word handled
+// with lane = query, so the label reads and atomics of a wavefront are contiguous 256-byte
+// rows.  Narrow labels: lane word x holds queries x*QPL .. x*QPL+QPL-1, so one row read (two
+// for 256 16-bit distances) covers all pending queries of the vertex.  Best when many queries are in flight together; the
+// two-pass kernel below is better when few are.
+template <int K, int LM>
+// 32-bit labels: 72 VGPRs at K <= 8 (7 waves per SIMD instead of 6 at 77, no spills): the row
+// reads are latency-bound, so the extra resident wave is worth more (-7 to -8% per pass on
+// RMAT-24).  Narrow labels: at most 64 VGPRs, 8 waves per SIMD (from 4-5 waves at 85-118
+// VGPRs: 25-40% faster passes on RMAT-24 x 256).
+__global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_plane_push_wave(
+    PlaneArgs args) {
     PlaneArgs a = args;
     plane_resolve(a);
     constexpr int TV = TILE_EDGES / K, EMAX = TV + 2, NW = WG / WAVE;
@@ -361,15 +435,21 @@ __global__ __launch_bounds__(WG, K <= 8 ? 7 : 1) void k_plane_push_wave(PlaneArg
         ctl->qpack[(a.it + 1) % 3] = 0;
         ctl->iters = a.it;
     }
-    const uint32_t ntiles = (T + TV - 1) / TV;
+    // Tile width: TV virtual nodes, or fewer when the frontier is small, so its virtual nodes
+    // spread over the whole grid (a wavefront takes one virtual node at a time, so a few
+    // full tiles would leave most of the GPU idle and the rest walking 64 nodes each).  A
+    // tile of tv virtual nodes meets at most tv + 2 entries (every entry has one or more).
+    const uint32_t tv = max((uint32_t)NW, min((uint32_t)TV, (T + gridDim.x - 1) / gridDim.x));
+    const int emax = min(EMAX, (int)tv + 2);
+    const uint32_t ntiles = (T + tv - 1) / tv;
     const uint32_t c0 = (uint32_t)((uint64_t)blockIdx.x * ntiles / gridDim.x);
     const uint32_t c1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
     if (c0 >= c1) return;
     const int wv = threadIdx.x / WAVE, lane = lane_id();
-    uint32_t e_start = wg_search(a.sin, nE, c0 * TV);
+    uint32_t e_start = wg_search(a.sin, nE, c0 * tv);
     for (uint32_t tile = c0; tile < c1; ++tile) {
-        const uint32_t t0 = tile * TV, t1 = min(t0 + (uint32_t)TV, T);
-        for (int k = threadIdx.x; k < EMAX; k += WG) {
+        const uint32_t t0 = tile * tv, t1 = min(t0 + tv, T);
+        for (int k = threadIdx.x; k < emax; k += WG) {
             const uint64_t i = (uint64_t)e_start + k;
             sS[k] = (i < nE) ? a.sin[i] : 0xffffffffu;
         }
@@ -379,8 +459,8 @@ __global__ __launch_bounds__(WG, K <= 8 ? 7 : 1) void k_plane_push_wave(PlaneArg
             sMisc[2] = 0;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < EMAX; k += WG) {
-            const uint32_t cur = sS[k], nxt = (k + 1 < EMAX) ? sS[k + 1] : 0xffffffffu;
+        for (int k = threadIdx.x; k < emax; k += WG) {
+            const uint32_t cur = sS[k], nxt = (k + 1 < emax) ? sS[k + 1] : 0xffffffffu;
             if (cur <= t1 - 1 && nxt > t1 - 1) sMisc[0] = k + 1;
             if (cur <= t1 && nxt > t1) sMisc[1] = k + 1;
         }
@@ -411,6 +491,77 @@ __global__ __launch_bounds__(WG, K <= 8 ? 7 : 1) void k_plane_push_wave(PlaneArg
             if ((uint32_t)lane < ne) {
                 my_u = a.col[sRow[k] + e0 + lane];
                 my_w = a.wgt[sRow[k] + e0 + lane];
+            }
+            if constexpr (LM != LM_U32) {
+                using P = Packed<LM>;
+                constexpr uint32_t QPL = P::QPL;     // queries per lane word
+                constexpr uint32_t GL = 64 / QPL;    // lane words per 64-query mask word
+                constexpr uint32_t GM = (1u << QPL) - 1;
+                uint32_t* plane = a.lab;
+                const uint32_t nl = a.B / QPL;       // lane words per row
+                for (uint32_t x0 = 0; x0 < nl; x0 += WAVE) {  // passes of 64 lane words
+                    const uint32_t x = x0 + (uint32_t)lane;
+                    uint32_t nib = 0;
+                    if (x < nl) nib = (uint32_t)(a.chg[(uint64_t)v * a.W + x / GL] >> (QPL * (x % GL))) & GM;
+                    if (!__ballot(nib != 0)) continue;  // the same answer in every lane
+                    const uint32_t lv = nib ? plane[(uint64_t)v * nl + x] : 0u;
+                    bool ovf = false;
+                    // destinations and weights are taken from the edge lanes again where
+                    // needed (outside divergent code) rather than kept in K registers each
+                    uint32_t ecur[K];
+#pragma unroll
+                    for (int e = 0; e < K; ++e) {  // every destination row read before any update
+                        const uint32_t u = __shfl(my_u, e, WAVE);
+                        ecur[e] = (nib && (uint32_t)e < ne) ? plane[(uint64_t)u * nl + x] : 0u;
+                    }
+                    uint32_t imp[K];
+#pragma unroll
+                    for (int e = 0; e < K; ++e) {
+                        imp[e] = 0;
+                        const uint32_t u = __shfl(my_u, e, WAVE);
+                        const uint32_t wt = __shfl(my_w, e, WAVE);
+                        if (nib && (uint32_t)e < ne) {
+                            const uint32_t cd = P::cand(lv, wt, nib, ovf);
+                            uint32_t c = ecur[e];
+                            uint32_t nv = P::merge(c, cd);
+                            if (nv != c) {
+                                uint32_t* p = &plane[(uint64_t)u * nl + x];
+                                while (true) {
+                                    const uint32_t old = atomicCAS(p, c, nv);
+                                    if (old == c) {
+                                        imp[e] = P::improved(c, cd);
+                                        break;
+                                    }
+                                    c = old;
+                                    nv = P::merge(c, cd);
+                                    if (nv == c) break;
+                                }
+                            }
+                        }
+                    }
+                    if (ovf) atomicOr(&ctl->err, PLANE_ERR_NARROW);
+#pragma unroll
+                    for (int e = 0; e < K; ++e) {
+                        const uint32_t u = __shfl(my_u, e, WAVE);
+                        if (!__ballot(imp[e] != 0)) continue;
+                        // only vertices with out-edges carry changes and are queued (each
+                        // queued entry owns >= 1 virtual node); a sink's label is final
+                        if (!a.vnc[u]) continue;
+                        // the improved queries of each 64-query word: its GL lane words
+                        // are neighbouring lanes
+                        uint64_t word = (uint64_t)imp[e] << (QPL * (x % GL));
+#pragma unroll
+                        for (uint32_t off = GL / 2; off >= 1; off >>= 1) word |= __shfl_xor(word, (int)off, WAVE);
+                        if ((x % GL) == 0 && word)
+                            atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + x / GL], (unsigned long long)word);
+                        if (lane == 0 && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+                            const uint32_t pos = atomicAdd(&sMisc[2], 1u);
+                            if (pos < TILE_EDGES) sQ[pos] = u;
+                            else atomicOr(&ctl->err, 16u);
+                        }
+                    }
+                }
+                continue;
             }
             for (uint32_t w = 0; w < a.W; ++w) {
                 const uint64_t m = a.chg[(uint64_t)v * a.W + w];
@@ -778,14 +929,35 @@ __global__ void k_plane_rotate(const uint32_t* q, const Ctl* ctl, int slot,
         atomicOr((unsigned long long*)&act[threadIdx.x], sAct[threadIdx.x]);
 }
 
-// out[q * n + u] = planes[perm[u] * B + q]: answers in query-major, original vertex order
+// out[q * n + u] = planes[perm[u] * B + q]: answers in query-major, original vertex order,
+// narrow labels widened (their all-ones INF to INF)
 __global__ void k_planes_out(const uint32_t* __restrict__ planes, const uint32_t* __restrict__ perm,
-                             uint32_t n, uint32_t B, uint32_t nq, uint32_t* __restrict__ out) {
+                             uint32_t n, uint32_t B, uint32_t nq, int lm, uint32_t* __restrict__ out) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * nq;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(i / n), u = (uint32_t)(i % n);
-        out[i] = planes[(uint64_t)perm[u] * B + q];
+        const uint64_t j = (uint64_t)perm[u] * B + q;
+        uint32_t x;
+        if (lm == LM_W8) {
+            x = reinterpret_cast<const uint8_t*>(planes)[j];
+            if (x == 0xffu) x = INF;
+        } else if (lm == LM_D16) {
+            x = reinterpret_cast<const uint16_t*>(planes)[j];
+            if (x == 0xffffu) x = INF;
+        } else {
+            x = planes[j];
+        }
+        out[i] = x;
     }
+}
+// Largest edge weight (atomicMax into *out, zeroed by the caller).
+__global__ void k_max_u32(const uint32_t* __restrict__ a, uint64_t n, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, a[i]);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, WAVE));
+    if (lane_id() == 0 && m) atomicMax(out, m);
 }
 
 // Fills with 16-byte stores where the base allows (the planes are many GB).
@@ -821,13 +993,16 @@ __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* _
 // source vertices with their virtual-node prefix, computed on the host.
 __global__ void k_seed(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ sq, uint32_t ns,
                        uint64_t* __restrict__ m1, uint64_t* __restrict__ m2, uint32_t* __restrict__ lab,
-                       uint32_t R, uint32_t B, uint32_t val) {
+                       uint32_t R, uint32_t B, uint32_t val, int lm) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
         const uint64_t idx = (uint64_t)sv[i] * R + sq[i] / 64;
         const uint64_t bit = 1ull << (sq[i] % 64);
         atomicOr((unsigned long long*)&m1[idx], (unsigned long long)bit);
         if (m2) atomicOr((unsigned long long*)&m2[idx], (unsigned long long)bit);
-        lab[(uint64_t)sv[i] * B + sq[i]] = val;
+        const uint64_t j = (uint64_t)sv[i] * B + sq[i];
+        if (lm == LM_W8) reinterpret_cast<uint8_t*>(lab)[j] = (uint8_t)(val == INF ? 0xffu : val);
+        else if (lm == LM_D16) reinterpret_cast<uint16_t*>(lab)[j] = (uint16_t)(val == INF ? 0xffffu : val);
+        else lab[j] = val;
     }
 }
 
@@ -938,6 +1113,7 @@ int cu_count_b(int device) {
 struct BatchWork {
     DevMem m1, m3;          // BFS: per-vertex mask records (m1); planes: chg (m1), nchg (m3)
     DevMem planes;          // n x B labels
+    int64_t wmax = -1;      // largest edge weight of the graph (-1: not measured yet)
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
     DevMem devit;           // replayed chunks: {iteration, stamp base}
     uint32_t epoch = 1;
@@ -974,14 +1150,29 @@ eeg_status add_node(hipGraph_t graph, hipGraphNode_t* prev, const void* fn, unsi
 
 // Graph of BATCH_CHUNK iterations (BFS: push, finalize; planes: push, two rotations; each
 // followed by a counter tick), built node by node and kept until its arguments change.
+// Dense plane push for K and label width lm.
+const void* dense_push(uint32_t K, int lm) {
+    switch (lm) {
+        case LM_W8:
+            return (K == 4) ? (const void*)k_plane_push_wave<4, LM_W8> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_W8>
+                   : (K == 16) ? (const void*)k_plane_push_wave<16, LM_W8> : (const void*)k_plane_push_wave<32, LM_W8>;
+        case LM_D16:
+            return (K == 4) ? (const void*)k_plane_push_wave<4, LM_D16> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_D16>
+                   : (K == 16) ? (const void*)k_plane_push_wave<16, LM_D16> : (const void*)k_plane_push_wave<32, LM_D16>;
+        default:
+            return (K == 4) ? (const void*)k_plane_push_wave<4, LM_U32> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_U32>
+                   : (K == 16) ? (const void*)k_plane_push_wave<16, LM_U32> : (const void*)k_plane_push_wave<32, LM_U32>;
+    }
+}
+
 eeg_status batch_graph(BatchWork& bw, bool bfs, uint32_t K, unsigned grid, const MsArgs& ma, const PlaneArgs& pa,
-                       uint32_t nsrc, hipGraphExec_t* out) {
+                       uint32_t nsrc, int lm, hipGraphExec_t* out) {
     const bool dense = nsrc > PLANE_DENSE_FROM;
-    std::vector<uint8_t> key(sizeof(MsArgs) + sizeof(PlaneArgs) + 16);
+    std::vector<uint8_t> key(sizeof(MsArgs) + sizeof(PlaneArgs) + 20);
     memcpy(key.data(), &ma, sizeof(MsArgs));
     memcpy(key.data() + sizeof(MsArgs), &pa, sizeof(PlaneArgs));
-    const uint32_t tail[4] = {bfs ? 1u : 0u, K, grid, dense ? 1u : 0u};
-    memcpy(key.data() + sizeof(MsArgs) + sizeof(PlaneArgs), tail, 16);
+    const uint32_t tail[5] = {bfs ? 1u : 0u, K, grid, dense ? 1u : 0u, (uint32_t)lm};
+    memcpy(key.data() + sizeof(MsArgs) + sizeof(PlaneArgs), tail, 20);
     if (bw.exec && bw.key == key) {
         *out = bw.exec;
         return EEG_OK;
@@ -1017,8 +1208,7 @@ eeg_status batch_graph(BatchWork& bw, bool bfs, uint32_t K, unsigned grid, const
     const void* fpush =
         bfs ? ((K == 4) ? (const void*)k_ms_push<4> : (K == 8) ? (const void*)k_ms_push<8>
                : (K == 16) ? (const void*)k_ms_push<16> : (const void*)k_ms_push<32>)
-        : dense ? ((K == 4) ? (const void*)k_plane_push_wave<4> : (K == 8) ? (const void*)k_plane_push_wave<8>
-                   : (K == 16) ? (const void*)k_plane_push_wave<16> : (const void*)k_plane_push_wave<32>)
+        : dense ? dense_push(K, lm)
                 : ((K == 4) ? (const void*)k_plane_push<4> : (K == 8) ? (const void*)k_plane_push<8>
                    : (K == 16) ? (const void*)k_plane_push<16> : (const void*)k_plane_push<32>);
     hipGraphNode_t prev = nullptr;
@@ -1058,8 +1248,31 @@ static BatchWork& batch_work(eeg_graph* g) {
     return *static_cast<BatchWork*>(g->batch_state.get());
 }
 
-eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
-                     void* out, bool out_device, eeg_stats* st) {
+namespace {
+
+// Largest edge weight of the graph, measured once per handle.
+eeg_status graph_wmax(eeg_graph* g, BatchWork& bw, uint32_t* wmax) {
+    if (bw.wmax < 0) {
+        DevMem d;
+        EEG_TRY(d.alloc(8));
+        uint32_t h = 0;
+        EEG_HIP(hipMemsetAsync(d.p, 0, 4, g->stream));
+        if (g->m) {
+            k_max_u32<<<grid_n(g->m), WG, 0, g->stream>>>(g->out.wp(), g->m, d.as<uint32_t>());
+            EEG_HIP(hipGetLastError());
+        }
+        EEG_HIP(hipMemcpyAsync(&h, d.p, 4, hipMemcpyDeviceToHost, g->stream));
+        EEG_HIP(hipStreamSynchronize(g->stream));
+        bw.wmax = h;
+    }
+    *wmax = (uint32_t)bw.wmax;
+    return EEG_OK;
+}
+
+// One multi-source pass with labels of width lm.  A narrow pass whose distances outgrow its
+// labels sets *overflow and returns EEG_OK without answers.
+eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                      void* out, bool out_device, eeg_stats* st, int lm, bool* overflow) {
     const auto t_host0 = std::chrono::steady_clock::now();
     if (nsrc == 0 || nsrc > 64 * MAXW) return fail(EEG_ERR_INVALID_ARG, "nsrc must be 1..%u per pass", 64 * MAXW);
     if (algo != EEG_BFS && algo != EEG_SSSP && algo != EEG_SSWP)
@@ -1085,7 +1298,8 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     const uint64_t nrec = (uint64_t)n * R;
     EEG_TRY(bw.m1.alloc(nrec * 8 + 8));
     if (!bfs) EEG_TRY(bw.m3.alloc(nw * 8 + 8));
-    EEG_TRY(bw.planes.alloc(nb * 4 + 4));
+    const uint64_t lbytes = lm == LM_W8 ? 1 : lm == LM_D16 ? 2 : 4;
+    EEG_TRY(bw.planes.alloc(nb * lbytes + 4));
     for (int i = 0; i < 2; ++i) {
         EEG_TRY(bw.q[i].alloc((size_t)n * 4 + 4));
         EEG_TRY(bw.s[i].alloc((size_t)n * 4 + 4));
@@ -1141,11 +1355,14 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     uint64_t* rec = bw.m1.as<uint64_t>();
     k_fill_u64<<<grid_n(nrec), WG, 0, s>>>(rec, nrec);
     if (!bfs) k_fill_u64<<<grid_n(nw), WG, 0, s>>>(bw.m3.as<uint64_t>(), nw);
-    k_fill_u32b<<<grid_n(nb), WG, 0, s>>>(bw.planes.as<uint32_t>(), nb, widest ? 0u : INF);
+    // every label unreached: 0 (widest), INF or all ones (narrow INF) for distances; nb is a
+    // multiple of 64, so the narrow planes are whole 32-bit words
+    const uint32_t unreached = widest ? 0u : (lm == LM_U32 ? INF : 0xffffffffu);
+    k_fill_u32b<<<grid_n(nb * lbytes / 4), WG, 0, s>>>(bw.planes.as<uint32_t>(), nb * lbytes / 4, unreached);
     EEG_HIP(hipGetLastError());
     // BFS: seen and visit get the source bits, level 0.  Planes: chg gets them.
     k_seed<<<grid_n(nsrc), WG, 0, s>>>(dseed, dseed + nsrc, nsrc, rec, bfs ? rec + 2 * W : nullptr,
-                                       bw.planes.as<uint32_t>(), R, B, widest ? INF : 0u);
+                                       bw.planes.as<uint32_t>(), R, B, widest ? INF : 0u, lm);
     k_set_frontier<<<grid_n(fvv.size() ? fvv.size() : 1), WG, 0, s>>>(
         dseed + 2 * nsrc, dseed + 3 * nsrc, (uint32_t)fvv.size(),
         ((unsigned long long)acc << 32) | (unsigned long long)fvv.size(), bw.q[0].as<uint32_t>(),
@@ -1215,7 +1432,7 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         ma.sa = pa.sa = bw.s[0].as<uint32_t>();
         ma.qb = pa.qb = bw.q[1].as<uint32_t>();
         ma.sb = pa.sb = bw.s[1].as<uint32_t>();
-        EEG_TRY(batch_graph(bw, bfs, K, grid, ma, pa, nsrc, &exec));
+        EEG_TRY(batch_graph(bw, bfs, K, grid, ma, pa, nsrc, lm, &exec));
     }
     uint32_t j = 1;
     while (true) {
@@ -1244,11 +1461,18 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
                 a.sin = bw.s[cur].as<uint32_t>();
                 a.qout = bw.q[cur ^ 1].as<uint32_t>();
                 a.sout = bw.s[cur ^ 1].as<uint32_t>();
-                EEG_TRY(launch_plane_iteration(a, K, grid, s, nsrc));
+                EEG_TRY(launch_plane_iteration(a, K, grid, s, nsrc, lm));
             }
         }
         EEG_HIP(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
         EEG_HIP(hipStreamSynchronize(s));
+        if (hc.err == PLANE_ERR_NARROW && lm != LM_U32) {  // distances outgrew the labels
+            bw.epoch = base + j + 2;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *overflow = true;
+            return EEG_OK;
+        }
         if (hc.err) return fail(EEG_ERR_DEVICE, "batched frontier overflow (err=%u)", hc.err);
         if (hc.done) break;
         level = hc.iters;
@@ -1268,7 +1492,7 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
             dst = tmp.as<uint32_t>();
         }
         k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B,
-                                                        nsrc, dst);
+                                                        nsrc, lm, dst);
         EEG_HIP(hipGetLastError());
         if (!out_device) EEG_HIP(hipMemcpyAsync(out, dst, outn * 4, hipMemcpyDeviceToHost, s));
         EEG_HIP(hipStreamSynchronize(s));
@@ -1284,7 +1508,36 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         st->launches = launches;
         st->ms_traversal = ms;
         st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+        st->label_bits = bfs ? 0u : (uint32_t)(8 * lbytes);
     }
+    return EEG_OK;
+}
+
+}  // namespace
+
+eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                     void* out, bool out_device, eeg_stats* st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int lm = LM_U32;
+    if ((algo == EEG_SSSP || algo == EEG_SSWP) && g->weighted && nsrc > PLANE_DENSE_FROM && nsrc <= 64 * MAXW) {
+        if (o.label_bits != 0 && o.label_bits != 32)
+            return fail(EEG_ERR_INVALID_ARG, "label_bits must be 0 or 32 (got %u)", o.label_bits);
+        if (o.label_bits == 0) {
+            if (algo == EEG_SSSP) {
+                lm = LM_D16;
+            } else {
+                uint32_t wmax = 0;
+                EEG_TRY(graph_wmax(g, batch_work(g), &wmax));
+                if (wmax <= 254) lm = LM_W8;
+            }
+        }
+    }
+    bool overflow = false;
+    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, lm, &overflow));
+    if (!overflow) return EEG_OK;
+    overflow = false;
+    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, LM_U32, &overflow));
+    if (st) st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return EEG_OK;
 }
 
@@ -1299,12 +1552,12 @@ eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s) {
     return EEG_OK;
 }
 
-eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight) {
+eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight, int lm) {
     void* args[] = {&a};
-    auto fn = (K == 4) ? k_plane_push<4> : (K == 8) ? k_plane_push<8> : (K == 16) ? k_plane_push<16> : k_plane_push<32>;
-    if (in_flight > PLANE_DENSE_FROM)
-        fn = (K == 4) ? k_plane_push_wave<4> : (K == 8) ? k_plane_push_wave<8> : (K == 16) ? k_plane_push_wave<16> : k_plane_push_wave<32>;
-    EEG_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(WG), args, 0, s));
+    const void* fn = (K == 4) ? (const void*)k_plane_push<4> : (K == 8) ? (const void*)k_plane_push<8>
+                     : (K == 16) ? (const void*)k_plane_push<16> : (const void*)k_plane_push<32>;
+    if (in_flight > PLANE_DENSE_FROM) fn = dense_push(K, lm);
+    EEG_HIP(hipLaunchKernel(fn, dim3(grid), dim3(WG), args, 0, s));
     // forget the pushed frontier's masks, then move this iteration's improvements in
     k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr,
                                         nullptr, nullptr, nullptr, nullptr);

@@ -74,7 +74,10 @@ eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s);
 // One plane iteration: push, then the two change-mask rotations.  `in_flight` (queries being
 // traversed) picks the push variant: one wavefront per virtual node with lane = query for
 // many, a two-pass kernel with K lanes per virtual node for few.
-eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight);
+// `lm` is the label width of the dense push (0: 32-bit labels, lane = query; narrow widths
+// only with the planes of eeg_run_batch).
+eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStream_t s, uint32_t in_flight,
+                                  int lm = 0);
 
 // Admission between iterations (one workgroup): pairs (pv[i], pc[i]) = (internal source,
 // column) get label `val`; their column bit is ORed into m_always, and into m_edges when the

@@ -383,6 +383,7 @@ extern "C" eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t*
         st->iterations = std::max(st->iterations, ps.iterations);
         st->launches += ps.launches;
         st->ms_traversal += ps.ms_traversal;
+        st->label_bits = std::max(st->label_bits, ps.label_bits);
     }
     if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

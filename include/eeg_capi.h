@@ -99,7 +99,12 @@ typedef struct eeg_opts {
                                   PageRank gather 48, in place 24) */
     uint32_t energy;           /* read the amd-smi energy counter before and after the run into
                                   stats->energy_j (two counter reads, ~0.25 ms each; 0 = off) */
-    uint32_t reserved[6];
+    uint32_t label_bits;       /* batched SSSP/SSWP (eeg_run_batch, more than 16 sources): label
+                                  width of the planes.  0 = the narrowest that holds the answers:
+                                  8-bit widest-path labels when every weight is <= 254, 16-bit
+                                  distances (a pass whose distances reach 65535 is rerun with
+                                  32-bit labels); 32 = 32-bit labels.  Answers are the same. */
+    uint32_t reserved[5];
 } eeg_opts;
 
 typedef struct eeg_stats {
@@ -119,7 +124,10 @@ typedef struct eeg_stats {
     double energy_j;            /* device energy over the run with opts.energy (else 0, or if unavailable) */
     uint32_t* activity;         /* optional caller buffer for per-iteration frontier sizes */
     uint32_t activity_cap;      /* capacity of activity */
-    uint32_t reserved[7];
+    uint32_t label_bits;        /* batched SSSP/SSWP: label width of the passes that answered
+                                   (8, 16 or 32, the widest over passes of 256 sources; 0 for
+                                   other runs) */
+    uint32_t reserved[6];
 } eeg_stats;
 
 typedef struct eeg_graph_info {

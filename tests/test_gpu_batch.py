@@ -149,3 +149,78 @@ def test_batched_ppr_planes_against_single_and_fp64(rmat13, nsrc):
         if q < 4:
             single, _ = g.run("ppr", int(x), variant="sync_push_dd")
             assert np.abs(lab[q].astype(np.float64) - single.astype(np.float64)).sum() < 1e-6
+
+
+@pytest.mark.parametrize("alg,bits", [("sssp", 16), ("sswp", 8)])
+@pytest.mark.parametrize("nsrc", [17, 37, 256])
+@pytest.mark.parametrize("K", [4, 8, 16, 32])
+def test_batch_narrow_labels_equal_32_bit(rmat13, alg, bits, nsrc, K):
+    """More than 16 sources take narrow label planes (16-bit distances; 8-bit widest-path
+    labels, every weight here <= 64); their answers are the 32-bit planes' bit for bit."""
+    n, s, d, w, g = rmat13
+    srcs = np.random.default_rng(nsrc + K).integers(0, n, nsrc).astype(np.uint32)
+    srcs[0] = int(np.argmax(np.bincount(s, minlength=n)))
+    out, st = g.run_batch(alg, srcs, K=K)
+    assert st["label_bits"] == bits
+    wide, stw = g.run_batch(alg, srcs, K=K, label_bits=32)
+    assert stw["label_bits"] == 32
+    assert np.array_equal(out, wide)
+    for q in (0, nsrc - 1):
+        assert np.array_equal(out[q], O.reference_labels(ALG[alg], n, s, d, w, int(srcs[q]))), (alg, q)
+
+
+def _chain(total):
+    """A path 0 -> 1 -> ... -> 9 whose weights sum to `total`, plus a side edge."""
+    n = 12
+    s = np.array(list(range(9)) + [10], np.uint32)
+    d = np.array(list(range(1, 10)) + [11], np.uint32)
+    w = np.full(10, total // 9, np.uint32)
+    w[8] += total - int(w[:9].sum())
+    w[9] = 3
+    return n, s, d, w
+
+
+@pytest.mark.parametrize("total,bits", [(65534, 16), (65535, 32), (200000, 32)])
+def test_batch_distances_past_16_bits_rerun_with_32(total, bits):
+    """The largest 16-bit distance is 65534 (65535 means unreached); a pass whose distances reach
+    65535 is answered again with 32-bit labels, and every answer equals the oracle's."""
+    n, s, d, w = _chain(total)
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    try:
+        srcs = np.array([0] * 10 + [1, 2, 3, 4, 5, 6, 7, 8, 10, 11], np.uint32)
+        out, st = g.run_batch("sssp", srcs)
+        assert st["label_bits"] == bits
+        assert out[0][9] == total
+        for q, x in enumerate(srcs):
+            assert np.array_equal(out[q], O.reference_labels(O.SSSP, n, s, d, w, int(x))), q
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("wmax,bits", [(254, 8), (255, 32)])
+def test_batch_widest_labels_need_weights_below_255(wmax, bits):
+    """8-bit widest-path labels hold weights up to 254 (255 stands for the source's INF); a
+    graph with a heavier edge takes 32-bit labels.  Answers equal the oracle's either way."""
+    n = 64
+    rng = np.random.default_rng(wmax)
+    s = rng.integers(0, n, 400).astype(np.uint32)
+    d = rng.integers(0, n, 400).astype(np.uint32)
+    w = rng.integers(1, wmax + 1, 400).astype(np.uint32)
+    w[0] = wmax
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    try:
+        srcs = np.arange(40, dtype=np.uint32)
+        out, st = g.run_batch("sswp", srcs)
+        assert st["label_bits"] == bits
+        for q, x in enumerate(srcs):
+            ref = O.reference_labels(O.SSWP, n, s, d, w, int(x))
+            assert np.array_equal(out[q], ref), q
+            assert out[q][x] == ref[x]  # the source keeps INF
+    finally:
+        g.close()
+
+
+def test_batch_label_bits_must_be_0_or_32(rmat13):
+    n, s, d, w, g = rmat13
+    with pytest.raises(eg.EEGError):
+        g.run_batch("sssp", np.arange(20, dtype=np.uint32), label_bits=16)
