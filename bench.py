@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--serve-qps", default="25,50,100,200,400,1000,5000",
                    help="offered loads (queries/s) of the serving sweep")
     p.add_argument("--serve-queries", type=int, default=200, help="queries per serving point")
+    p.add_argument("--serve-ppr-columns", type=int, default=0,
+                   help="PPR engine width of the serving section (8, 16 or 32; 0 = the server's default)")
     p.add_argument("--serve-queries-top", type=int, default=600,
                    help="queries at the highest offered load (the saturated throughput point)")
     p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp"),
@@ -516,7 +518,7 @@ def serving_section(eg, a):
     one_j = round(p1[0] * (t2 - t1) / nmax, 4) if p1 else None
     log(f"one at a time: {nmax} queries in {t2 - t1:.2f} s ({nmax / (t2 - t1):.1f} q/s)")
 
-    srv = eg.Server(g, discard_labels=True)
+    srv = eg.Server(g, discard_labels=True, ppr_columns=a.serve_ppr_columns)
     for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
         srv.wait(srv.submit(alg, 0))
     points = []

@@ -316,7 +316,7 @@ size_t PprPlane::bytes_for(uint32_t n, uint32_t B, uint32_t nspill) {
 }
 
 eeg_status PprPlane::init(eeg_graph* g, uint32_t Bc, float d, hipStream_t s) {
-    if (Bc != 16 && Bc != 32) return fail(EEG_ERR_INVALID_ARG, "PPR planes hold 16 or 32 columns");
+    if (Bc != 8 && Bc != 16 && Bc != 32) return fail(EEG_ERR_INVALID_ARG, "PPR planes hold 8, 16 or 32 columns");
     B = Bc;
     n = g->n;
     damping = d;
@@ -361,7 +361,11 @@ eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipS
     a.damping = damping;
     a.cols = cols;
     double* fix_part = res_part.as<double>() + (size_t)grid_pull * B;
-    if (B == 16) {
+    if (B == 8) {
+        k_pps_pull<8><<<grid_pull, WG, 0, s>>>(a);
+        k_pps_fix<8><<<grid_fix, WG, 0, s>>>(a, fix_part);
+        k_pps_reduce<8><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
+    } else if (B == 16) {
         k_pps_pull<16><<<grid_pull, WG, 0, s>>>(a);
         k_pps_fix<16><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<16><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
@@ -377,7 +381,10 @@ eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipS
 eeg_status PprPlane::extract(eeg_graph* g, uint32_t j, uint32_t q, uint32_t src, float* out, hipStream_t s) {
     const int wb = (int)(j & 1u);
     const unsigned gr = grid_of(n, 8192);
-    if (B == 16)
+    if (B == 8)
+        k_pps_extract<8><<<gr, WG, 0, s>>>(x[wb].as<float2>(), g->in.rowp(), g->outdeg.as<uint32_t>(),
+                                           g->perm.as<uint32_t>(), n, q, src, 1.0f - damping, out);
+    else if (B == 16)
         k_pps_extract<16><<<gr, WG, 0, s>>>(x[wb].as<float2>(), g->in.rowp(), g->outdeg.as<uint32_t>(),
                                             g->perm.as<uint32_t>(), n, q, src, 1.0f - damping, out);
     else
@@ -400,7 +407,7 @@ eeg_status run_ppr_batch(eeg_graph* g, const uint32_t* sources, uint32_t nsrc, c
     EEG_TRY(ensure_in(g));
     EEG_TRY(ensure_ppr_seg(g));
     PprPlane pl;
-    EEG_TRY(pl.init(g, nsrc > 16 ? 32 : 16, d, g->stream));
+    EEG_TRY(pl.init(g, nsrc > 16 ? 32 : nsrc > 8 ? 16 : 8, d, g->stream));
     const uint32_t B = pl.B;
     hipStream_t s = g->stream;
     DevMem dout;

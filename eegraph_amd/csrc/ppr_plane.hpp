@@ -40,7 +40,8 @@ struct PprPlane {
     eeg_status extract(eeg_graph* g, uint32_t j, uint32_t q, uint32_t src, float* out, hipStream_t s);
 };
 
-// eeg_run_batch for PPR: passes of up to 32 restart vertices; row q of out_host (n floats, may
+// eeg_run_batch for PPR: planes of 8, 16 or 32 columns (the narrowest that holds nsrc, 32 for
+// more), columns refilled as queries converge; row q of out_host (n floats, may
 // be null) receives the ranks of sources[q].
 eeg_status run_ppr_batch(eeg_graph* g, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o, float* out_host,
                          eeg_stats* st);

@@ -896,10 +896,13 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         if (c.first_iteration) e->it = c.first_iteration;
         EEG_TRY(e->init(g));
     }
-    // PPR engine: rank planes of cfg.ppr_columns (16 or 32, default 32) columns, 16 when 32 do
-    // not fit a third of the free memory
-    if (c.ppr_columns != 0 && c.ppr_columns != 16 && c.ppr_columns != 32)
-        return fail(EEG_ERR_INVALID_ARG, "ppr_columns must be 16 or 32");
+    // PPR engine: rank planes of cfg.ppr_columns (8, 16 or 32, default 16) columns, fewer when
+    // those do not fit a third of the free memory.  Default from the RMAT-26 load sweep (served
+    // over one-at-a-time queries/s at 100 / 400 / 5000 offered q/s): 8 columns 1.49 / 1.79 /
+    // 2.10, 16 columns 1.30 / 1.55 / 2.33, 32 columns 0.94 / 1.07 / 2.45 - a query's latency is
+    // its ~67 passes, and a pass of 32 columns costs ~3x one of 8.
+    if (c.ppr_columns != 0 && c.ppr_columns != 8 && c.ppr_columns != 16 && c.ppr_columns != 32)
+        return fail(EEG_ERR_INVALID_ARG, "ppr_columns must be 8, 16 or 32");
     if ((c.flags & EEG_SERVER_NO_PPR_ENGINE) == 0 && n > 0) {
         {
             std::lock_guard<std::mutex> lk(g->mu);
@@ -909,9 +912,10 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
         size_t fr = 0, tot = 0;
         EEG_HIP(hipMemGetInfo(&fr, &tot));
         const uint32_t nsp = g->pprseg.nspill;
-        const uint32_t want = c.ppr_columns ? c.ppr_columns : 32u;
-        const uint32_t B = (want == 32 && PprPlane::bytes_for(n, 32, nsp) <= fr / 3) ? 32u
-                           : PprPlane::bytes_for(n, 16, nsp) <= fr / 3 ? 16u : 0u;
+        const uint32_t want = c.ppr_columns ? c.ppr_columns : 16u;
+        uint32_t B = 0;
+        for (uint32_t b = want; b >= 8 && !B; b /= 2)
+            if (PprPlane::bytes_for(n, b, nsp) <= fr / 3) B = b;
         if (B) {
             PprEngine* e = new (std::nothrow) PprEngine();
             if (!e) return fail(EEG_ERR_OOM, "host allocation failed");

@@ -300,9 +300,10 @@ typedef struct eeg_server_cfg {
     uint32_t first_iteration;    /* testing: the traversal engines' iteration counter at start (0 = 1);
                                     from 0xf0000000 on an engine admits nothing until it drains,
                                     then resets its columns and restarts the counter at 1 */
-    uint32_t ppr_columns;        /* PPR engine width: 16 or 32 restart vertices per gather pass
-                                    (0 = 32; a pass of 16 costs ~0.6x one of 32, for lower
-                                    latency when few PPR queries are in flight) */
+    uint32_t ppr_columns;        /* PPR engine width: 8, 16 or 32 restart vertices per gather
+                                    pass (0 = 16).  A pass of 8 / 16 costs about 0.3x / 0.6x
+                                    one of 32: narrower engines answer sooner, wider ones serve
+                                    more PPR queries per second when many are in flight */
     uint32_t reserved[3];
 } eeg_server_cfg;
 /* Answers are not kept: columns are given back as soon as a query converges and wait() reports
@@ -312,8 +313,8 @@ typedef struct eeg_server_cfg {
 #define EEG_SERVER_PLANE_ENGINES 2u
 /* BFS goes through lanes instead of the multi-source BFS engine. */
 #define EEG_SERVER_NO_BFS_ENGINE 4u
-/* PPR goes through lanes instead of the PPR engine (rank planes of 16 or 32 restart vertices,
- * one gather pass per iteration for every query in flight). */
+/* PPR goes through lanes instead of the PPR engine (rank planes of 8, 16 or 32 restart
+ * vertices, one gather pass per iteration for every query in flight). */
 #define EEG_SERVER_NO_PPR_ENGINE 8u
 typedef struct eeg_query_result {
     uint32_t algo;

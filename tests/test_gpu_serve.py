@@ -176,10 +176,11 @@ def test_engine_counter_restarts_near_its_limit(rmat12):
                 _check(g, alg, int(x), lab, n, row, col, singles)
 
 
-@pytest.mark.parametrize("columns", [16, 32])
+@pytest.mark.parametrize("columns", [8, 16, 32])
 def test_ppr_engine_columns_join_and_leave(rmat12, columns):
     """More PPR queries than the engine has columns, submitted while others run: each answer
-    within L1 1e-6 of its single-query run and of the FP64 oracle; queries share iterations."""
+    within L1 1e-6 of its single-query run and of the FP64 oracle; queries share iterations.
+    columns = 0 is the default width (16)."""
     n, s, d, w, g, row, col = rmat12
     rng = np.random.default_rng(17)
     srcs = [int(x) for x in rng.integers(0, n, size=70)]
