@@ -95,8 +95,8 @@ typedef struct eeg_opts {
     double epsilon;            /* PR and PPR: stop when L1(x_k+1 - x_k) < epsilon (0 = 1e-7) */
     uint32_t wgs_per_cu;       /* workgroups per compute unit of the tile kernels' grid
                                   (0 = default: 24 for frontier queues and topology sweeps, 40
-                                  for in-place sweeps, 48 for flag scans and batched passes;
-                                  PageRank gather 48, in place 24) */
+                                  for in-place sweeps, 48 for flag scans and batched passes (96
+                                  with narrow labels); PageRank gather 48, in place 24) */
     uint32_t energy;           /* read the amd-smi energy counter before and after the run into
                                   stats->energy_j (two counter reads, ~0.25 ms each; 0 = off) */
     uint32_t label_bits;       /* batched SSSP/SSWP (eeg_run_batch, more than 16 sources): label
