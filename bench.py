@@ -266,7 +266,7 @@ def sweep_section(g, a, src, m):
 
 
 def autotune_section(g, src, default_K):
-    """eeg_autotune per algorithm against the default options (sync_push_dd, compact queues,
+    """eeg_autotune per algorithm against the default options (sync_push_dd, sorted queues,
     K = 8, default grid): median device traversal ms of 5 runs each, same source."""
     import eegraph_amd as eg
 
@@ -407,8 +407,9 @@ def packed_tuned_section(eg, gs, psrc):
     import eegraph_amd as eg_
 
     out = {}
-    forms = [dict(variant="sync_push_dd"), dict(variant="sync_push_dd", frontier="flags"),
-             dict(variant="async_push_dd"), dict(variant="async_push_td")]
+    forms = [dict(variant="sync_push_dd"), dict(variant="sync_push_dd", frontier="compact"),
+             dict(variant="sync_push_dd", frontier="flags"), dict(variant="async_push_dd"),
+             dict(variant="async_push_td")]
     for alg in ("bfs", "sssp", "sswp", "cc"):
         tuned = [x.autotune(alg, [int(y)])[0] for x, y in zip(gs, psrc)]
         for x, y, o in zip(gs, psrc, tuned):
@@ -455,9 +456,10 @@ def batched_only(eg, a):
 
 def grid_section(eg, a):
     """High-diameter case: 4096 x 4096 grid, 4-neighbour edges both ways plus 1% random
-    shortcut pairs (seed 3), w ~ U{1..1000}, source 0.  BFS/SSSP/SSWP/CC with compacted
-    frontier queues against flag scans (sync_push_dd), and the async variants; device
-    traversal time of one query each (after one untimed run)."""
+    shortcut pairs (seed 3), w ~ U{1..1000}, source 0.  BFS/SSSP/SSWP/CC with frontier queues
+    (sorted: rebuilt from a bitmap in vertex order; compact: in improvement order) against
+    flag scans (sync_push_dd), and the async variants; device traversal time of one query
+    each (after one untimed run)."""
     t = time.perf_counter()
     s, d, w = eg.gen_grid(4096, 4096, 0.01, seed=3, wmin=1, wmax=1000)
     g = eg.Graph.from_edges(s, d, w, n=4096 * 4096)
@@ -467,7 +469,8 @@ def grid_section(eg, a):
     res = {"workload": "grid4096x4096 + 1% shortcuts, seed 3, w 1..1000, source 0", "edges": m}
     for alg in ("bfs", "sssp", "sswp", "cc"):
         r = {}
-        for name, var, fr in (("sync_compact", "sync_push_dd", "compact"), ("sync_flags", "sync_push_dd", "flags"),
+        for name, var, fr in (("sync_sorted", "sync_push_dd", "sorted"), ("sync_compact", "sync_push_dd", "compact"),
+                              ("sync_flags", "sync_push_dd", "flags"),
                               ("async_push_dd", "async_push_dd", "compact"), ("async_push_td", "async_push_td", "compact")):
             c = g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False, count_work=True)[1]
             _, st = g.run(alg, 0, variant=var, frontier=fr, K=a.K, want_labels=False)

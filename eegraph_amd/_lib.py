@@ -41,7 +41,7 @@ ALGOS = {"bfs": BFS, "sssp": SSSP, "sswp": SSWP, "cc": CC, "pr": PR, "ppr": PPR}
 ASYNC_PUSH_TD, ASYNC_PUSH_DD, SYNC_PUSH_TD, SYNC_PUSH_DD = range(4)
 VARIANTS = {"async_push_td": ASYNC_PUSH_TD, "async_push_dd": ASYNC_PUSH_DD,
             "sync_push_td": SYNC_PUSH_TD, "sync_push_dd": SYNC_PUSH_DD}
-FRONTIER_COMPACT, FRONTIER_FLAGS = 0, 1
+FRONTIER_SORTED, FRONTIER_FLAGS, FRONTIER_COMPACT = 0, 1, 2
 
 PREP_TIGR, PREP_SYM, PREP_TRANSPOSE = 1, 2, 4
 DIST_INFINITY = 4294967294

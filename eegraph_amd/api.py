@@ -76,14 +76,15 @@ def save_edgelist(path, n, src, dst, w=None):
     L.check(L.lib.eeg_save_edgelist(str(path).encode(), C.byref(el)), "eeg_save_edgelist")
 
 
-def make_opts(variant="sync_push_dd", K=8, frontier="compact", max_iters=0, timing=False,
+def make_opts(variant="sync_push_dd", K=8, frontier="sorted", max_iters=0, timing=False,
               count_work=False, collect_activity=False, damping=0.85, epsilon=0.0, acc=0.0,
               wgs_per_cu=0, energy=False, block=0, label_bits=0):
     o = L.Opts()
     o.variant = L.VARIANTS[variant] if isinstance(variant, str) else int(variant)
     o.K = K
     o.block = block
-    o.frontier = {"compact": L.FRONTIER_COMPACT, "flags": L.FRONTIER_FLAGS}[frontier] \
+    o.frontier = {"compact": L.FRONTIER_COMPACT, "flags": L.FRONTIER_FLAGS,
+                  "sorted": L.FRONTIER_SORTED}[frontier] \
         if isinstance(frontier, str) else int(frontier)
     o.max_iters = max_iters
     o.timing = 1 if timing else 0

@@ -82,7 +82,7 @@ extern "C" eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* 
     base.epsilon = best->epsilon;
     base.max_iters = best->max_iters;
     base.variant = EEG_SYNC_PUSH_DD;
-    base.frontier = EEG_FRONTIER_COMPACT;
+    base.frontier = EEG_FRONTIER_SORTED;
     base.K = 8;
 
     // Every layout the search touches is built before any timing.
@@ -130,6 +130,8 @@ extern "C" eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* 
         eeg_opts o = base;
         if (algo != EEG_PR) {
             o.frontier = EEG_FRONTIER_FLAGS;
+            cand.push_back(o);
+            o.frontier = EEG_FRONTIER_COMPACT;
             cand.push_back(o);
         }
         o = base;

@@ -107,6 +107,7 @@ struct Work {
     DevMem q[2], s[2];  // frontier queues: entry vertex ids and vnode prefix
     DevMem stamp;       // u32[n] iteration stamps for queue dedup
     DevMem flag;        // u32[n] iteration stamps for flag-scan frontiers
+    DevMem bitmap;      // u32[(n+31)/32]: vertices improved this iteration (sorted frontiers)
     DevMem vnpush;      // u32[P] last label pushed by each virtual node (async dd)
     DevMem vdirty;      // u32 per 64-vertex block: stamp of the last improvement in the block
     DevMem ctl;         // Ctl

@@ -86,10 +86,11 @@ bool ArgumentParser::Parse() {
         } else if (strcmp(f, "--wgs") == 0) {
             wgsPerCU = (unsigned)atoi(v);
         } else if (strcmp(f, "--frontier") == 0) {
-            if (strcmp(v, "flags") == 0) flagFrontier = true;
-            else if (strcmp(v, "compact") == 0) flagFrontier = false;
+            if (strcmp(v, "sorted") == 0) frontierKind = EEG_FRONTIER_SORTED;
+            else if (strcmp(v, "flags") == 0) frontierKind = EEG_FRONTIER_FLAGS;
+            else if (strcmp(v, "compact") == 0) frontierKind = EEG_FRONTIER_COMPACT;
             else {
-                std::cout << "Frontier must be compact or flags\n";
+                std::cout << "Frontier must be sorted, compact or flags\n";
                 exit(0);
             }
         } else if (strcmp(f, "--autotune") == 0) {
@@ -133,7 +134,7 @@ std::string ArgumentParser::GenerateHelpString() {
     str += "\n    [--activity]: sum node activity vector in results(Default false) E.g. --activity true";
     str += "\n    [--k]: Virtual-node degree bound 4, 8, 16 or 32 (Default: 8). E.g. --k 16";
     str += "\n    [--wgs]: Workgroups per compute unit of the tile kernels (Default: auto). E.g. --wgs 16";
-    str += "\n    [--frontier]: Frontier of sync_push_dd, compact or flags (Default: compact)";
+    str += "\n    [--frontier]: Frontier of sync_push_dd, sorted, compact or flags (Default: sorted)";
     str += "\n    [--autotune]: Pick K, workgroups and variant by timing short runs. E.g. --autotune true";
     str += "\n\n";
     return str;

@@ -146,7 +146,7 @@ Result eegraph_run(uint32_t algo, ArgumentParser& arguments, Graph& graph) {
     eeg_opts o{};
     o.variant = arguments.variant;
     o.K = arguments.K;
-    o.frontier = arguments.flagFrontier ? EEG_FRONTIER_FLAGS : EEG_FRONTIER_COMPACT;
+    o.frontier = arguments.frontierKind;
     o.wgs_per_cu = arguments.wgsPerCU;
     o.collect_activity = arguments.nodeActivity ? 1 : 0;
     if (algo == EEG_PR && arguments.hasAcc) o.epsilon = arguments.acc;

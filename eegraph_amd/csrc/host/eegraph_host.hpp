@@ -91,7 +91,7 @@ public:
     // additions
     unsigned int K = Part_Size;
     unsigned int wgsPerCU = 0;
-    bool flagFrontier = false;
+    unsigned int frontierKind = 0;  // eeg_frontier_kind of sync_push_dd (0 = sorted)
     bool autotune = false;
     std::string algorithm;  // for the combined `eegraph` executable
 

@@ -43,6 +43,8 @@ struct RelaxArgs {
     uint32_t* stamp;
     uint32_t* flag;
     uint32_t* vnpush;
+    uint32_t* bitmap;   // sorted frontiers (M_QUEUE): improved vertices set their bit here and
+                        // k_compact_frontier builds the next queue; null: queued by the kernel
     Ctl* ctl;
     uint32_t n_static;  // entries / vnodes when qslot_in < 0
     uint32_t P_static;
@@ -284,19 +286,29 @@ __global__ __launch_bounds__(BS) void k_relax(RelaxArgs args) {
             }
             // Phase 4: frontier bookkeeping.  The queue needs each improved vertex's
             // virtual-node count and a dedup stamp; both are issued for all rounds first.
+            // Sorted frontiers only set the vertex's bit (an OR whose old value nobody waits
+            // for); counts, dedup and order come from k_compact_frontier.
             uint32_t env[ROUNDS];
             if constexpr (MODE == M_QUEUE) {
+                if (a.bitmap) {
 #pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? a.vnc[ed[r]] : 0u;
-                uint32_t eso[ROUNDS];
+                    for (int r = 0; r < ROUNDS; ++r) {
+                        env[r] = 0u;
+                        if (eu[r]) atomicOr(&a.bitmap[ed[r] >> 5], 1u << (ed[r] & 31u));
+                    }
+                } else {
 #pragma unroll
-                for (int r = 0; r < ROUNDS; ++r) {
-                    eso[r] = a.stampv;
-                    if (env[r]) eso[r] = atomicExch(&a.stamp[ed[r]], a.stampv);
+                    for (int r = 0; r < ROUNDS; ++r) env[r] = eu[r] ? a.vnc[ed[r]] : 0u;
+                    uint32_t eso[ROUNDS];
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r) {
+                        eso[r] = a.stampv;
+                        if (env[r]) eso[r] = atomicExch(&a.stamp[ed[r]], a.stampv);
+                    }
+#pragma unroll
+                    for (int r = 0; r < ROUNDS; ++r)
+                        if (eso[r] == a.stampv) env[r] = 0;
                 }
-#pragma unroll
-                for (int r = 0; r < ROUNDS; ++r)
-                    if (eso[r] == a.stampv) env[r] = 0;
             }
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
@@ -556,6 +568,94 @@ KernelFn pick_kernel(int alg, uint32_t K, int mode, uint32_t block) {
     }
 }
 
+// Sorted frontier: the vertices whose bit iteration `it` set, with out-edges, as the next
+// queue (buffer it & 1, Ctl slot it % 3), in ascending id order within each block's 8192-vertex
+// chunk; chunks take their queue ranges in the order their atomic lands.  Bits are cleared as
+// they are read.  One thread per 32-bit word; replayed graphs resolve `it` from dev_it.
+struct CompactArgs {
+    uint32_t* bitmap;
+    uint32_t nwords;
+    uint32_t n;
+    const uint32_t* vnc;
+    Ctl* ctl;
+    uint32_t *qa, *sa, *qb, *sb;  // queue buffers 0 and 1
+    uint32_t cap;
+    uint32_t it;
+    const uint32_t* dev_it;
+    uint32_t it_off;
+};
+
+__global__ __launch_bounds__(WG) void k_compact_frontier(CompactArgs a) {
+    const uint32_t it = a.dev_it ? *a.dev_it + a.it_off : a.it;
+    Ctl* ctl = a.ctl;
+    if (ctl->done) return;
+    uint32_t* qout = (it & 1u) ? a.qb : a.qa;
+    uint32_t* sout = (it & 1u) ? a.sb : a.sa;
+    const int slot = (int)(it % 3u);
+    __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ uint32_t sBase[2];
+    for (uint32_t c0 = blockIdx.x * WG; c0 < a.nwords; c0 += gridDim.x * WG) {
+        const uint32_t wi = c0 + threadIdx.x;
+        const uint32_t word = wi < a.nwords ? a.bitmap[wi] : 0u;
+        if (!__syncthreads_or(word != 0u)) continue;  // the same answer in every thread
+        // the word's 32 virtual-node counts, read once: one 128-byte line as 8 vector loads
+        // (a last word past n reads only the counts that exist)
+        uint32_t cv[32];
+        if (word) {
+            a.bitmap[wi] = 0u;
+            const uint32_t v0 = wi * 32u;
+            if (v0 + 32u <= a.n) {
+                const uint4* p = reinterpret_cast<const uint4*>(a.vnc + v0);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint4 x = p[k];
+                    cv[4 * k] = x.x;
+                    cv[4 * k + 1] = x.y;
+                    cv[4 * k + 2] = x.z;
+                    cv[4 * k + 3] = x.w;
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < 32; ++b) cv[b] = (v0 + (uint32_t)b < a.n) ? a.vnc[v0 + b] : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 32; ++b) cv[b] = 0u;
+        }
+        uint32_t cnt = 0, vs = 0;
+#pragma unroll
+        for (int b = 0; b < 32; ++b) {
+            const bool on = ((word >> b) & 1u) && cv[b];
+            cnt += on ? 1u : 0u;
+            vs += on ? cv[b] : 0u;
+        }
+        uint64_t tot;
+        const uint64_t pre = block_excl_scan<uint64_t>(((uint64_t)vs << 32) | cnt, sScan, &tot);
+        if (threadIdx.x == 0) {
+            const unsigned long long old = tot ? atomicAdd(&ctl->qpack[slot], (unsigned long long)tot) : 0ull;
+            sBase[0] = (uint32_t)(old & 0xffffffffull);
+            sBase[1] = (uint32_t)(old >> 32);
+        }
+        __syncthreads();
+        uint32_t e = sBase[0] + (uint32_t)(pre & 0xffffffffull);
+        uint32_t vv = sBase[1] + (uint32_t)(pre >> 32);
+#pragma unroll
+        for (int b = 0; b < 32; ++b) {
+            // vertices without out-edges are left out: their labels are final, nothing to push
+            if (!((word >> b) & 1u) || !cv[b]) continue;
+            if (e < a.cap) {
+                qout[e] = wi * 32u + (uint32_t)b;
+                sout[e] = vv;
+            } else {
+                atomicOr(&ctl->err, 2u);
+            }
+            ++e;
+            vv += cv[b];
+        }
+        __syncthreads();  // sBase is rewritten by the next chunk
+    }
+}
+
 bool block_ok(uint32_t block) { return block == 0 || block == 256 || block == 512 || block == 1024; }
 
 int cu_count(int device) {
@@ -607,12 +707,18 @@ constexpr uint32_t GRAPH_CHUNK = 6;
 // on first use and kept in the work state with the last few others, so queries that
 // alternate algorithms reuse theirs.  It holds kernel nodes only.
 eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, unsigned block, const RelaxArgs& ga, hipStream_t s,
-                       hipGraphExec_t* out) {
-    std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid) + sizeof(block));
+                       hipGraphExec_t* out, const CompactArgs* ca = nullptr, unsigned grid_c = 0) {
+    const size_t kc = ca ? sizeof(CompactArgs) + sizeof(grid_c) : 0;
+    std::vector<uint8_t> key(sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid) + sizeof(block) + kc);
     memcpy(key.data(), &ga, sizeof(RelaxArgs));
     memcpy(key.data() + sizeof(RelaxArgs), &kfn, sizeof(kfn));
     memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn), &grid, sizeof(grid));
     memcpy(key.data() + sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid), &block, sizeof(block));
+    if (ca) {
+        uint8_t* kp = key.data() + sizeof(RelaxArgs) + sizeof(kfn) + sizeof(grid) + sizeof(block);
+        memcpy(kp, ca, sizeof(CompactArgs));
+        memcpy(kp + sizeof(CompactArgs), &grid_c, sizeof(grid_c));
+    }
     for (size_t i = 0; i < w.graphs.size(); ++i) {
         if (w.graphs[i].key == key) {
             Work::GraphEntry hit = std::move(w.graphs[i]);
@@ -628,7 +734,8 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, unsigned block, con
     }
     // built node by node (no stream capture, which would interact with other threads'
     // HIP calls): GRAPH_CHUNK relaxations in one dependency chain, node i running iteration
-    // dev_it[0] + i, then one tick that advances dev_it[0] by GRAPH_CHUNK
+    // dev_it[0] + i (each followed by its frontier compaction for sorted frontiers), then one
+    // tick that advances dev_it[0] by GRAPH_CHUNK
     hipGraph_t graph = nullptr;
     EEG_HIP(hipGraphCreate(&graph, 0));
     RelaxArgs a = ga;
@@ -648,6 +755,21 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, unsigned block, con
         hipGraphNode_t nr = nullptr;
         e = hipGraphAddKernelNode(&nr, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
         prev = nr;
+        if (ca && e == hipSuccess) {
+            CompactArgs cx = *ca;
+            cx.it_off = i;
+            void* cargs[] = {&cx};
+            hipKernelNodeParams cp{};
+            cp.func = (void*)k_compact_frontier;
+            cp.gridDim = dim3(grid_c);
+            cp.blockDim = dim3(WG);
+            cp.sharedMemBytes = 0;
+            cp.kernelParams = cargs;
+            cp.extra = nullptr;
+            hipGraphNode_t nc = nullptr;
+            e = hipGraphAddKernelNode(&nc, graph, &prev, 1, &cp);
+            prev = nc;
+        }
     }
     uint32_t step = GRAPH_CHUNK;
     void* targs[] = {&itp, &step};
@@ -693,6 +815,8 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         case EEG_SYNC_PUSH_DD: mode = (o.frontier == EEG_FRONTIER_FLAGS) ? M_FLAGS : M_QUEUE; break;
         default: return fail(EEG_ERR_INVALID_ARG, "unknown variant %u", o.variant);
     }
+    if (o.frontier > EEG_FRONTIER_COMPACT) return fail(EEG_ERR_INVALID_ARG, "unknown frontier kind %u", o.frontier);
+    const bool sorted = mode == M_QUEUE && o.frontier == EEG_FRONTIER_SORTED;
     const int alg = (int)algo;  // EEG_BFS..EEG_CC map onto A_BFS..A_CC
     if ((alg == A_SSSP || alg == A_SSWP) && !g->weighted)
         return fail(EEG_ERR_INVALID_ARG, "SSSP/SSWP need a weighted graph");
@@ -715,6 +839,13 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         }
     }
     const uint32_t max_iters = o.max_iters ? o.max_iters : (n + 2);
+    const uint32_t nwords = (n + 31) / 32;
+    if (sorted) {
+        EEG_TRY(w.bitmap.alloc((size_t)nwords * 4 + 4));
+        // a run that stopped early (error, iteration limit) may have left bits behind
+        k_fill_u32<<<grid_for(nwords, 8192), WG, 0, s>>>(w.bitmap.as<uint32_t>(), nwords, 0u);
+        EEG_HIP(hipGetLastError());
+    }
     if (mode == M_QUEUE && !w.stamp.p) {
         EEG_TRY(w.stamp.alloc((size_t)n * 4 + 4));
         EEG_HIP(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, s));
@@ -812,6 +943,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
     ra.stamp = w.stamp.as<uint32_t>();
     ra.flag = w.flag.as<uint32_t>();
     ra.vnpush = w.vnpush.as<uint32_t>();
+    ra.bitmap = sorted ? w.bitmap.as<uint32_t>() : nullptr;
     ra.ctl = ctl;
     ra.cap_out = n;
     ra.n_static = t.nz;
@@ -823,6 +955,22 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         ra.vdirty = w.vdirty.as<uint32_t>();
         ra.tbase = base;
         ra.all_first = seeded ? 0u : 1u;
+    }
+
+    CompactArgs ca{};
+    unsigned grid_c = 1;
+    if (sorted) {
+        ca.bitmap = w.bitmap.as<uint32_t>();
+        ca.nwords = nwords;
+        ca.n = n;
+        ca.vnc = t.vnc.as<uint32_t>();
+        ca.ctl = ctl;
+        ca.qa = w.q[0].as<uint32_t>();
+        ca.sa = w.s[0].as<uint32_t>();
+        ca.qb = w.q[1].as<uint32_t>();
+        ca.sb = w.s[1].as<uint32_t>();
+        ca.cap = n;
+        grid_c = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(((uint64_t)nwords + WG - 1) / WG, (uint64_t)cus * 4));
     }
 
     // Event pool: per launch L (it = L+1) an "after" event at 2L+1 and, with timing, a
@@ -859,6 +1007,11 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         }
         void* args[] = {&ra};
         EEG_HIP(hipLaunchKernel((const void*)kfn, dim3(grid), dim3(block), args, 0, s));
+        if (sorted) {
+            ca.it = it;
+            void* cargs[] = {&ca};
+            EEG_HIP(hipLaunchKernel((const void*)k_compact_frontier, dim3(grid_c), dim3(WG), cargs, 0, s));
+        }
         EEG_TRY(w.event(2 * (size_t)it, &ea));
         EEG_HIP(hipEventRecord(ea, s));
         return EEG_OK;
@@ -914,7 +1067,10 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         ga.qb = w.q[1].as<uint32_t>();
         ga.sb = w.s[1].as<uint32_t>();
         hipGraphExec_t exec = nullptr;
-        EEG_TRY(chunk_graph(w, kfn, grid_queue, block, ga, s, &exec));
+        CompactArgs gca = ca;
+        gca.it = 0;
+        gca.dev_it = dit;
+        EEG_TRY(chunk_graph(w, kfn, grid_queue, block, ga, s, &exec, sorted ? &gca : nullptr, grid_c));
         EEG_TRY(w.poll_events());
         hipEvent_t* poll = w.poll;
         uint32_t replays = 0;
@@ -1046,8 +1202,11 @@ eeg_status run_traversal_fixed(eeg_graph* g, uint32_t algo, uint32_t source, con
         case EEG_ASYNC_PUSH_TD:
         case EEG_SYNC_PUSH_TD: mode = M_TD; break;
         case EEG_SYNC_PUSH_DD:
+            // sorted frontiers keep a bitmap: the compact queue stands in (same answers)
             if (o.frontier == EEG_FRONTIER_FLAGS)
                 return fail(EEG_ERR_UNSUPPORTED, "flag-scan frontiers are not available without allocation");
+            if (o.frontier > EEG_FRONTIER_COMPACT)
+                return fail(EEG_ERR_INVALID_ARG, "unknown frontier kind %u", o.frontier);
             mode = M_QUEUE;
             break;
         case EEG_ASYNC_PUSH_DD:

@@ -72,8 +72,13 @@ typedef enum eeg_variant {
 } eeg_variant;
 
 typedef enum eeg_frontier_kind {
-    EEG_FRONTIER_COMPACT = 0, /* ballot/prefix compacted vertex queue */
-    EEG_FRONTIER_FLAGS = 1    /* per-vertex flag scan over every virtual node */
+    EEG_FRONTIER_SORTED = 0,  /* (default) vertex queue in ascending id order per 8192-vertex
+                                 block, built from a 1-bit-per-vertex bitmap after every
+                                 iteration; eeg_run_device_noalloc runs EEG_FRONTIER_COMPACT
+                                 for it (same answers; the bitmap would need memory) */
+    EEG_FRONTIER_FLAGS = 1,   /* per-vertex flag scan over every virtual node */
+    EEG_FRONTIER_COMPACT = 2  /* vertex queue in the order vertices improve (ballot/prefix
+                                 compaction per tile, dedup stamps) */
 } eeg_frontier_kind;
 
 typedef struct eeg_graph eeg_graph;   /* opaque: device-resident CSR + Tigr tables */

@@ -57,6 +57,14 @@ def test_queue_activity_is_the_level_histogram_of_expanding_vertices(rmat14, K):
     assert st["activity_len"] >= len(want)
 
 
+@pytest.mark.parametrize("K", [4, 32])
+def test_sorted_queue_activity_is_the_level_histogram_too(rmat14, K):
+    n, s, d, w, g, src, lev, outdeg = rmat14
+    lab, st = g.run("bfs", src, variant="sync_push_dd", frontier="sorted", K=K, activity_cap=4096)
+    assert np.array_equal(lab, lev)
+    assert _trim(st["activity"]) == _trim(_level_hist(lev, outdeg > 0))
+
+
 def test_flag_scan_activity_counts_each_level_once(rmat14):
     n, s, d, w, g, src, lev, outdeg = rmat14
     lab, st = g.run("bfs", src, variant="sync_push_dd", frontier="flags", K=8, activity_cap=4096)

@@ -105,11 +105,12 @@ def test_rmat22_all_variants_and_K(rmat22, alg, variant, K):
     assert_same(lab, refs[alg], f"rmat22 {alg} {variant} K={K}")
 
 
+@pytest.mark.parametrize("frontier", ["flags", "sorted"])
 @pytest.mark.parametrize("alg", ["bfs", "sssp", "sswp", "cc"])
-def test_rmat22_flag_scans(rmat22, alg):
+def test_rmat22_flag_scans(rmat22, alg, frontier):
     g, n, src, refs, _ = rmat22
-    lab, _ = g.run(alg, src, variant="sync_push_dd", frontier="flags", K=8)
-    assert_same(lab, refs[alg], f"rmat22 {alg} flags")
+    lab, _ = g.run(alg, src, variant="sync_push_dd", frontier=frontier, K=8)
+    assert_same(lab, refs[alg], f"rmat22 {alg} {frontier}")
 
 
 @pytest.fixture(scope="module")
@@ -148,6 +149,7 @@ def grid4096():
 
 
 GRID_FORMS = {"compact": dict(variant="sync_push_dd", frontier="compact"),
+              "sorted": dict(variant="sync_push_dd", frontier="sorted"),
               "flags": dict(variant="sync_push_dd", frontier="flags"),
               "async_push_dd": dict(variant="async_push_dd"),
               "async_push_td": dict(variant="async_push_td")}

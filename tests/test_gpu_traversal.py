@@ -88,6 +88,34 @@ def test_flag_scan_frontier(rmat14, refs14, alg):
     assert np.array_equal(lab, refs[alg])
 
 
+@pytest.mark.parametrize("alg", list(ALG))
+@pytest.mark.parametrize("K", [4, 8, 16, 32])
+def test_sorted_frontier(rmat14, refs14, alg, K):
+    """Queues rebuilt from a bitmap in vertex order give the oracle's labels, and the same
+    iteration count and frontier sizes as the compacted queue (the same vertices per level)."""
+    n, s, d, w, g = rmat14
+    src, refs = refs14
+    lab, st = g.run(alg, src, variant="sync_push_dd", K=K, frontier="sorted", count_work=True)
+    assert np.array_equal(lab, refs[alg]), f"{alg} K={K}"
+    if alg == "bfs":  # BFS levels: every queue holds exactly one level's expanding vertices
+        # (entries staged per tile depend on the order: a vertex crossing a tile boundary is
+        # staged by both tiles, so those counts are not compared)
+        _, sa = g.run(alg, src, variant="sync_push_dd", K=K, frontier="sorted", activity_cap=4096)
+        _, sc = g.run(alg, src, variant="sync_push_dd", K=K, frontier="compact", activity_cap=4096, count_work=True)
+        assert st["iterations"] == sc["iterations"]
+        assert st["edges_relaxed"] == sc["edges_relaxed"]
+        assert np.array_equal(sa["activity"], sc["activity"])
+    for wpc in (1, 7):  # other grids, and with timing (one launch per iteration, no replay)
+        lab2, _ = g.run(alg, src, variant="sync_push_dd", K=K, frontier="sorted", wgs_per_cu=wpc, timing=True)
+        assert np.array_equal(lab2, refs[alg]), f"{alg} K={K} wpc={wpc}"
+
+
+def test_unknown_frontier_kind_is_rejected(rmat14):
+    n, s, d, w, g = rmat14
+    with pytest.raises(eg.EEGError):
+        g.run("bfs", 0, variant="sync_push_dd", frontier=3)
+
+
 def test_device_rmat_equals_host_rmat():
     s, d, w = eg.gen_rmat(12, 8, seed=5, wmin=1, wmax=64)
     gh = eg.Graph.from_edges(s, d, w, n=1 << 12)
@@ -103,7 +131,7 @@ def test_grid_with_shortcuts(alg):
     g = eg.Graph.from_edges(s, d, w, n=n)
     ref = O.reference_labels(ALG[alg], n, s, d, w, 0)
     for variant in VARIANTS:
-        for frontier in ("compact", "flags"):
+        for frontier in ("compact", "flags", "sorted"):
             lab, st = g.run(alg, 0, variant=variant, K=4, frontier=frontier)
             assert np.array_equal(lab, ref), f"{alg} {variant} {frontier}"
 
@@ -117,7 +145,8 @@ def test_static_sweeps_on_a_larger_grid(alg):
     g = eg.Graph.from_edges(s, d, w, n=n)
     src = 12345
     ref = O.reference_labels(ALG[alg], n, s, d, w, src)
-    for variant, frontier in (("async_push_dd", "compact"), ("sync_push_dd", "flags"), ("sync_push_dd", "compact")):
+    for variant, frontier in (("async_push_dd", "compact"), ("sync_push_dd", "flags"), ("sync_push_dd", "compact"),
+                              ("sync_push_dd", "sorted")):
         lab, st = g.run(alg, src, variant=variant, K=8, frontier=frontier)
         assert np.array_equal(lab, ref), f"{alg} {variant} {frontier}"
     g.close()

@@ -84,5 +84,8 @@ def test_noalloc_rejects_graphs_above_its_limit_and_unsupported_variants():
     o = eg.make_opts(variant="async_push_dd", K=8)
     rc = L.lib.eeg_run_device_noalloc(g.handle, L.BFS, 0, C.byref(o), out.p, None, None)
     assert rc == L.ERR_UNSUPPORTED
+    o = eg.make_opts(variant="sync_push_dd", frontier="flags", K=8)  # flags per vertex
+    rc = L.lib.eeg_run_device_noalloc(g.handle, L.BFS, 0, C.byref(o), out.p, None, None)
+    assert rc == L.ERR_UNSUPPORTED
     out.free()
     g.close()
