@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "frontier.hpp"
 
 namespace eeg {
 namespace {
@@ -554,10 +555,14 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                         for (uint32_t off = GL / 2; off >= 1; off >>= 1) word |= __shfl_xor(word, (int)off, WAVE);
                         if ((x % GL) == 0 && word)
                             atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + x / GL], (unsigned long long)word);
-                        if (lane == 0 && atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
-                            const uint32_t pos = atomicAdd(&sMisc[2], 1u);
-                            if (pos < TILE_EDGES) sQ[pos] = u;
-                            else atomicOr(&ctl->err, 16u);
+                        if (lane == 0) {
+                            if (a.bitmap) {
+                                atomicOr(&a.bitmap[u >> 5], 1u << (u & 31u));
+                            } else if (atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+                                const uint32_t pos = atomicAdd(&sMisc[2], 1u);
+                                if (pos < TILE_EDGES) sQ[pos] = u;
+                                else atomicOr(&ctl->err, 16u);
+                            }
                         }
                     }
                 }
@@ -600,7 +605,9 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                         // only vertices with out-edges carry changes and are queued (each
                         // queued entry owns >= 1 virtual node); a sink's label is final
                         atomicOr((unsigned long long*)&a.nchg[(uint64_t)u * a.W + w], (unsigned long long)um);
-                        if (atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
+                        if (a.bitmap) {
+                            atomicOr(&a.bitmap[u >> 5], 1u << (u & 31u));
+                        } else if (atomicExch(&a.stamp[u], a.stampv) != a.stampv) {
                             const uint32_t pos = atomicAdd(&sMisc[2], 1u);
                             if (pos < TILE_EDGES) sQ[pos] = u;
                             else atomicOr(&ctl->err, 16u);
@@ -1113,6 +1120,7 @@ int cu_count_b(int device) {
 struct BatchWork {
     DevMem m1, m3;          // BFS: per-vertex mask records (m1); planes: chg (m1), nchg (m3)
     DevMem planes;          // n x B labels
+    DevMem bitmap;          // dense plane passes: vertices improved this iteration (1 bit each)
     int64_t wmax = -1;      // largest edge weight of the graph (-1: not measured yet)
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
     DevMem devit;           // replayed chunks: {iteration, stamp base}
@@ -1128,6 +1136,22 @@ namespace {
 
 // Iterations per replayed chunk (a multiple of 2 and 3: queue ping-pong and control slots).
 constexpr uint32_t BATCH_CHUNK = 6;
+
+// Compaction of a dense plane pass's bitmap into the queue the push would have written.
+CompactArgs compact_args(const PlaneArgs& a) {
+    CompactArgs ca{};
+    ca.bitmap = a.bitmap;
+    ca.nwords = (a.n + 31) / 32;
+    ca.n = a.n;
+    ca.vnc = a.vnc;
+    ca.ctl = a.ctl;
+    ca.cap = a.qcap;
+    return ca;
+}
+unsigned compact_grid(uint32_t n) {
+    const uint64_t words = ((uint64_t)n + 31) / 32;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + WG - 1) / WG, 1024));
+}
 
 __global__ void k_tick_b(uint32_t* it) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *it += 1;
@@ -1204,6 +1228,17 @@ eeg_status batch_graph(BatchWork& bw, bool bfs, uint32_t K, unsigned grid, const
     const uint32_t* rqb = pa.qb;
     uint64_t* ractb = pa.act;
     void* r0args[] = {&rq, &rctl, &rslot, &rnone, &rto, &rW, &move0, &ract, &rit, &rqa, &rqb, &ractb};
+    // sorted queues: the iteration comes from the device counter (ticked after each iteration)
+    CompactArgs cga{};
+    if (pa.bitmap) {
+        cga = compact_args(pa);
+        cga.qa = pa.qa;
+        cga.sa = pa.sa;
+        cga.qb = pa.qb;
+        cga.sb = pa.sb;
+        cga.dev_it = pa.dev_it;
+    }
+    void* cargs[] = {&cga};
     void* r1args[] = {&rq, &rctl, &rslot, &rfrom1, &rto, &rW, &move1, &ract, &rit, &rqa, &rqb, &ractb};
     const void* fpush =
         bfs ? ((K == 4) ? (const void*)k_ms_push<4> : (K == 8) ? (const void*)k_ms_push<8>
@@ -1219,6 +1254,8 @@ eeg_status batch_graph(BatchWork& bw, bool bfs, uint32_t K, unsigned grid, const
             if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_ms_finalize, grid, WG, margs);
         } else {
             rc = add_node(graph, &prev, fpush, grid, WG, pargs);
+            if (rc == EEG_OK && pa.bitmap)
+                rc = add_node(graph, &prev, (const void*)k_compact_frontier, compact_grid(pa.n), WG, cargs);
             if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_plane_rotate, 2048, WG, r0args);
             if (rc == EEG_OK) rc = add_node(graph, &prev, (const void*)k_plane_rotate, 2048, WG, r1args);
         }
@@ -1420,6 +1457,13 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
         pa.n = n;
         pa.qcap = qcap;
         pa.widest = widest ? 1 : 0;
+        if (nsrc > PLANE_DENSE_FROM) {  // dense passes: sorted queues rebuilt from a bitmap
+            const uint64_t nwords = ((uint64_t)n + 31) / 32;
+            EEG_TRY(bw.bitmap.alloc(nwords * 4 + 4));
+            k_fill_u32b<<<grid_n(nwords), WG, 0, s>>>(bw.bitmap.as<uint32_t>(), nwords, 0u);
+            EEG_HIP(hipGetLastError());
+            pa.bitmap = bw.bitmap.as<uint32_t>();
+        }
     }
     const bool replay = !o.timing;
     hipGraphExec_t exec = nullptr;
@@ -1560,6 +1604,14 @@ eeg_status launch_plane_iteration(PlaneArgs a, uint32_t K, unsigned grid, hipStr
                      : (K == 16) ? (const void*)k_plane_push<16> : (const void*)k_plane_push<32>;
     if (in_flight > PLANE_DENSE_FROM) fn = dense_push(K, lm);
     EEG_HIP(hipLaunchKernel(fn, dim3(grid), dim3(WG), args, 0, s));
+    if (a.bitmap) {  // sorted queue: the push only set bits
+        CompactArgs ca = compact_args(a);
+        ca.qa = ca.qb = a.qout;
+        ca.sa = ca.sb = a.sout;
+        ca.it = a.it;
+        void* cargs[] = {&ca};
+        EEG_HIP(hipLaunchKernel((const void*)k_compact_frontier, dim3(compact_grid(a.n)), dim3(WG), cargs, 0, s));
+    }
     // forget the pushed frontier's masks, then move this iteration's improvements in
     k_plane_rotate<<<2048, WG, 0, s>>>(a.qin, a.ctl, (int)((a.it - 1) % 3), nullptr, a.chg, a.W, 0, nullptr,
                                         nullptr, nullptr, nullptr, nullptr);

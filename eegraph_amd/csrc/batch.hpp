@@ -67,6 +67,9 @@ struct PlaneArgs {
     int widest;           // 0: SSSP (min, +w), 1: SSWP (max, min(.,w))
     const uint32_t* dev_it;  // replayed graph, as in MsArgs
     uint32_t *qa, *sa, *qb, *sb;
+    uint32_t* bitmap;     // dense passes of eeg_run_batch: improved vertices set their bit and
+                          // k_compact_frontier builds the next queue in vertex order (null: the
+                          // push appends to the queue itself)
 };
 
 // One BFS level: push then finalize (launch indices 2*level-1 and 2*level).
