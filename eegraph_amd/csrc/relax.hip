@@ -586,7 +586,7 @@ int cu_count(int device) {
 // Memory the traversal state needs beyond the graph (used by admission control too).
 size_t traversal_state_bytes(uint32_t n, uint64_t P, int mode) {
     size_t b = (size_t)n * 4 + sizeof(Ctl);
-    if (mode == M_QUEUE) b += (size_t)n * 4 * 5;
+    if (mode == M_QUEUE) b += (size_t)n * 4 * 5 + ((size_t)n + 31) / 32 * 4;  // queues, stamps, bitmap
     if (mode == M_FLAGS) b += (size_t)n * 4;
     if (mode == M_ASYNC) b += (size_t)P * 4;
     return b;
