@@ -36,6 +36,10 @@ def test_help_and_usage_errors_exit_zero_like_the_reference():
     assert rc == 0 and "energy file and/or energy stats" in out
     rc, out = run("bfs", "--input", "g.el", "--k", "7")
     assert rc == 0 and "K must be" in out
+    rc, out = run("bfs", "--input", "g.el", "--frontier", "bitmap")
+    assert rc == 0 and "Frontier must be sorted, compact or flags" in out
+    rc, out = run("bfs", "--help")
+    assert "sorted, compact or flags (Default: sorted)" in out
 
 
 def test_unreadable_graph_fails_with_status():
