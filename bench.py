@@ -326,6 +326,7 @@ def batched_section(eg, a):
                       "batched_p99_ms": round(bat * 1e3, 3), "iterations": st["iterations"]}
         if alg != "bfs":
             multi[alg]["label_bits"] = st["label_bits"]
+            multi[alg]["narrow_reruns"] = st["narrow_reruns"]
         log(f"batched {alg}: {multi[alg]}")
     res[f"rmat{a.batch_scale}_256_sources"] = multi
     # personalised PageRank: 64 restart vertices (the first 64 sources) as planes of 32
@@ -452,7 +453,7 @@ def batched_only(eg, a):
     print(json.dumps({"batched_only": a.batched_only, "workload": f"rmat{a.batch_scale}_ef16_seed4_256_sources",
                       "ms": round(bat * 1e3, 3), "ms_traversal": round(st["ms_traversal"], 3),
                       "queries_per_s": round(256 / bat, 1), "iterations": st["iterations"],
-                      "label_bits": st["label_bits"]}), flush=True)
+                      "label_bits": st["label_bits"], "narrow_reruns": st["narrow_reruns"]}), flush=True)
     g.close()
 
 

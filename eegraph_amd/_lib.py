@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("vnodes_processed", u64), ("vertices_touched", u64), ("label_updates", u64),
                 ("bytes_alg", C.c_double), ("pr_residual", C.c_double),
                 ("energy_j", C.c_double), ("activity", P(u32)), ("activity_cap", u32),
-                ("label_bits", u32), ("reserved", u32 * 6)]
+                ("label_bits", u32), ("narrow_reruns", u32), ("reserved", u32 * 5)]
 
     def to_dict(self):
         d = {}

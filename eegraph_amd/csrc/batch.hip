@@ -338,9 +338,17 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
 //   LM_D16  SSSP: u16 distances, 0xffff = INF.  A finite candidate >= 0xffff sets Ctl::err
 //           bit 64 and is dropped; labels below it are exact (a vertex's shortest path only
 //           passes smaller distances), and the caller reruns the pass on 32-bit planes.
+//   LM_D8   SSSP with small weights: u8 distances, 0xff = INF.  A finite candidate >= 0xff is
+//           dropped; when the destination is still unreached in that query the query's bit is
+//           set in PlaneArgs::ovq.  A query without the bit is exact: every vertex at distance
+//           <= 254 is found as above, and a vertex at a finite distance >= 255 would have a
+//           first such vertex y on its shortest path, whose predecessor x (distance <= 254,
+//           final and pushed) offers y a candidate >= 255 while y is unreached (a label is
+//           never below the true distance).  The caller answers the flagged queries again
+//           with wider labels.
 // Improvements of a lane word are compare-and-swap loops: labels only improve, so every
 // failed swap means another improvement landed and the loop ends.
-enum LabelMode : int { LM_U32 = 0, LM_W8 = 1, LM_D16 = 2 };
+enum LabelMode : int { LM_U32 = 0, LM_W8 = 1, LM_D16 = 2, LM_D8 = 3 };
 constexpr uint32_t PLANE_ERR_NARROW = 64u;
 
 template <int LM>
@@ -350,7 +358,7 @@ struct Packed<LM_W8> {
     static constexpr uint32_t QPL = 4, BITS = 8, FM = 0xffu;  // labels per 32-bit lane word
     __device__ static uint32_t f(uint32_t x, int i) { return (x >> (BITS * i)) & FM; }
     // candidates of the queries in `nib` over an edge of weight w (<= 254): min(label, w)
-    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, bool&) {
+    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t&) {
         uint32_t c = 0;
 #pragma unroll
         for (int i = 0; i < (int)QPL; ++i)
@@ -370,11 +378,13 @@ struct Packed<LM_W8> {
         return m;
     }
 };
-template <>
-struct Packed<LM_D16> {
-    static constexpr uint32_t QPL = 2, BITS = 16, FM = 0xffffu;
+// Distances of BITS bits per label, all ones = INF.  cand() sets bit i of `ov` for a query
+// whose finite candidate does not fit.
+template <uint32_t Q, uint32_t NB>
+struct PackedDist {
+    static constexpr uint32_t QPL = Q, BITS = NB, FM = (1u << NB) - 1u;
     __device__ static uint32_t f(uint32_t x, int i) { return (x >> (BITS * i)) & FM; }
-    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, bool& ovf) {
+    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
         uint32_t c = 0xffffffffu;
 #pragma unroll
         for (int i = 0; i < (int)QPL; ++i) {
@@ -382,7 +392,7 @@ struct Packed<LM_D16> {
             if (!((nib >> i) & 1u) || l == FM) continue;
             const uint64_t sm = (uint64_t)l + w;
             if (sm < FM) c = (c & ~(FM << (BITS * i))) | ((uint32_t)sm << (BITS * i));
-            else if (sm < INF) ovf = true;  // finite, but not representable
+            else if (sm < INF) ov |= 1u << i;  // finite, but not representable
         }
         return c;
     }
@@ -398,7 +408,17 @@ struct Packed<LM_D16> {
         for (int i = 0; i < (int)QPL; ++i) m |= (f(c, i) < f(cur, i) ? 1u : 0u) << i;
         return m;
     }
+    __device__ static uint32_t unreached(uint32_t cur) {  // labels still INF
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < (int)QPL; ++i) m |= (f(cur, i) == FM ? 1u : 0u) << i;
+        return m;
+    }
 };
+template <>
+struct Packed<LM_D16> : PackedDist<2, 16> {};
+template <>
+struct Packed<LM_D8> : PackedDist<4, 8> {};
 
 // Dense variant: one wavefront per virtual node.  LM_U32: every pending 64-query word handled
 // with lane = query, so the label reads and atomics of a wavefront are contiguous 256-byte
@@ -422,6 +442,7 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
     __shared__ uint32_t sQ[TILE_EDGES];
     __shared__ uint32_t sMisc[8];
     __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ uint64_t sOv[MAXW];  // 8-bit distances: flagged queries, which stop pushing
     Ctl* ctl = a.ctl;
     if (a.act && blockIdx.x == 0 && threadIdx.x < MAXW) a.act[(a.it % 4) * MAXW + threadIdx.x] = 0;
     if (ctl->done) return;
@@ -458,6 +479,11 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
             sMisc[0] = 1;
             sMisc[1] = 1;
             sMisc[2] = 0;
+        }
+        if constexpr (LM == LM_D8) {
+            // a flagged query is answered again wider, so its further pushes are wasted work
+            // (its column only feeds itself); a stale copy only costs some of that work
+            if (threadIdx.x < MAXW) sOv[threadIdx.x] = __builtin_nontemporal_load(&a.ovq[threadIdx.x]);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < emax; k += WG) {
@@ -504,9 +530,10 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                     const uint32_t x = x0 + (uint32_t)lane;
                     uint32_t nib = 0;
                     if (x < nl) nib = (uint32_t)(a.chg[(uint64_t)v * a.W + x / GL] >> (QPL * (x % GL))) & GM;
+                    if constexpr (LM == LM_D8) nib &= ~(uint32_t)(sOv[x / GL] >> (QPL * (x % GL)));
                     if (!__ballot(nib != 0)) continue;  // the same answer in every lane
                     const uint32_t lv = nib ? plane[(uint64_t)v * nl + x] : 0u;
-                    bool ovf = false;
+                    uint32_t ovf = 0;  // D16: candidates past 16 bits; D8: flagged queries
                     // destinations and weights are taken from the edge lanes again where
                     // needed (outside divergent code) rather than kept in K registers each
                     uint32_t ecur[K];
@@ -522,8 +549,14 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                         const uint32_t u = __shfl(my_u, e, WAVE);
                         const uint32_t wt = __shfl(my_w, e, WAVE);
                         if (nib && (uint32_t)e < ne) {
-                            const uint32_t cd = P::cand(lv, wt, nib, ovf);
+                            uint32_t ov = 0;
+                            const uint32_t cd = P::cand(lv, wt, nib, ov);
                             uint32_t c = ecur[e];
+                            if constexpr (LM == LM_D8) {
+                                if (ov) ovf |= ov & P::unreached(c);
+                            } else {
+                                ovf |= ov;
+                            }
                             uint32_t nv = P::merge(c, cd);
                             if (nv != c) {
                                 uint32_t* p = &plane[(uint64_t)u * nl + x];
@@ -540,7 +573,12 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                             }
                         }
                     }
-                    if (ovf) atomicOr(&ctl->err, PLANE_ERR_NARROW);
+                    if constexpr (LM == LM_D8) {
+                        if (ovf)
+                            atomicOr((unsigned long long*)&a.ovq[x / GL], (unsigned long long)ovf << (QPL * (x % GL)));
+                    } else if (ovf) {
+                        atomicOr(&ctl->err, PLANE_ERR_NARROW);
+                    }
 #pragma unroll
                     for (int e = 0; e < K; ++e) {
                         const uint32_t u = __shfl(my_u, e, WAVE);
@@ -945,7 +983,7 @@ __global__ void k_planes_out(const uint32_t* __restrict__ planes, const uint32_t
         const uint32_t q = (uint32_t)(i / n), u = (uint32_t)(i % n);
         const uint64_t j = (uint64_t)perm[u] * B + q;
         uint32_t x;
-        if (lm == LM_W8) {
+        if (lm == LM_W8 || lm == LM_D8) {
             x = reinterpret_cast<const uint8_t*>(planes)[j];
             if (x == 0xffu) x = INF;
         } else if (lm == LM_D16) {
@@ -1007,7 +1045,7 @@ __global__ void k_seed(const uint32_t* __restrict__ sv, const uint32_t* __restri
         atomicOr((unsigned long long*)&m1[idx], (unsigned long long)bit);
         if (m2) atomicOr((unsigned long long*)&m2[idx], (unsigned long long)bit);
         const uint64_t j = (uint64_t)sv[i] * B + sq[i];
-        if (lm == LM_W8) reinterpret_cast<uint8_t*>(lab)[j] = (uint8_t)(val == INF ? 0xffu : val);
+        if (lm == LM_W8 || lm == LM_D8) reinterpret_cast<uint8_t*>(lab)[j] = (uint8_t)(val == INF ? 0xffu : val);
         else if (lm == LM_D16) reinterpret_cast<uint16_t*>(lab)[j] = (uint16_t)(val == INF ? 0xffffu : val);
         else lab[j] = val;
     }
@@ -1122,6 +1160,8 @@ struct BatchWork {
     DevMem planes;          // n x B labels
     DevMem bitmap;          // dense plane passes: vertices improved this iteration (1 bit each)
     int64_t wmax = -1;      // largest edge weight of the graph (-1: not measured yet)
+    DevMem ovq;             // 8-bit SSSP passes: queries whose distances outgrew 8 bits
+    bool d8_off = false;    // a previous 8-bit SSSP pass flagged more than a quarter of its queries
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
     DevMem devit;           // replayed chunks: {iteration, stamp base}
     uint32_t epoch = 1;
@@ -1183,6 +1223,9 @@ const void* dense_push(uint32_t K, int lm) {
         case LM_D16:
             return (K == 4) ? (const void*)k_plane_push_wave<4, LM_D16> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_D16>
                    : (K == 16) ? (const void*)k_plane_push_wave<16, LM_D16> : (const void*)k_plane_push_wave<32, LM_D16>;
+        case LM_D8:
+            return (K == 4) ? (const void*)k_plane_push_wave<4, LM_D8> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_D8>
+                   : (K == 16) ? (const void*)k_plane_push_wave<16, LM_D8> : (const void*)k_plane_push_wave<32, LM_D8>;
         default:
             return (K == 4) ? (const void*)k_plane_push_wave<4, LM_U32> : (K == 8) ? (const void*)k_plane_push_wave<8, LM_U32>
                    : (K == 16) ? (const void*)k_plane_push_wave<16, LM_U32> : (const void*)k_plane_push_wave<32, LM_U32>;
@@ -1306,10 +1349,12 @@ eeg_status graph_wmax(eeg_graph* g, BatchWork& bw, uint32_t* wmax) {
     return EEG_OK;
 }
 
-// One multi-source pass with labels of width lm.  A narrow pass whose distances outgrow its
-// labels sets *overflow and returns EEG_OK without answers.
+// One multi-source pass with labels of width lm.  A 16-bit pass whose distances outgrow its
+// labels sets *overflow and returns EEG_OK without answers; an 8-bit pass answers every query
+// and returns the queries to answer again (bit q of ovq[q / 64]) in ovq_out.
 eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
-                      void* out, bool out_device, eeg_stats* st, int lm, bool* overflow) {
+                      void* out, bool out_device, eeg_stats* st, int lm, bool* overflow,
+                      uint64_t ovq_out[MAXW]) {
     const auto t_host0 = std::chrono::steady_clock::now();
     if (nsrc == 0 || nsrc > 64 * MAXW) return fail(EEG_ERR_INVALID_ARG, "nsrc must be 1..%u per pass", 64 * MAXW);
     if (algo != EEG_BFS && algo != EEG_SSSP && algo != EEG_SSWP)
@@ -1335,7 +1380,7 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
     const uint64_t nrec = (uint64_t)n * R;
     EEG_TRY(bw.m1.alloc(nrec * 8 + 8));
     if (!bfs) EEG_TRY(bw.m3.alloc(nw * 8 + 8));
-    const uint64_t lbytes = lm == LM_W8 ? 1 : lm == LM_D16 ? 2 : 4;
+    const uint64_t lbytes = (lm == LM_W8 || lm == LM_D8) ? 1 : lm == LM_D16 ? 2 : 4;
     EEG_TRY(bw.planes.alloc(nb * lbytes + 4));
     for (int i = 0; i < 2; ++i) {
         EEG_TRY(bw.q[i].alloc((size_t)n * 4 + 4));
@@ -1353,6 +1398,11 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
     }
     EEG_TRY(bw.ctl.alloc(sizeof(Ctl)));
     EEG_TRY(bw.seeds.alloc((size_t)4 * nsrc * 4 + 16));
+    if (lm == LM_D8) {
+        EEG_TRY(bw.ovq.alloc(MAXW * 8));
+        k_fill_u64<<<1, 64, 0, s>>>(bw.ovq.as<uint64_t>(), MAXW);
+        EEG_HIP(hipGetLastError());
+    }
     Ctl* ctl = bw.ctl.as<Ctl>();
     EEG_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
 
@@ -1457,6 +1507,7 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
         pa.n = n;
         pa.qcap = qcap;
         pa.widest = widest ? 1 : 0;
+        pa.ovq = lm == LM_D8 ? bw.ovq.as<uint64_t>() : nullptr;
         if (nsrc > PLANE_DENSE_FROM) {  // dense passes: sorted queues rebuilt from a bitmap
             const uint64_t nwords = ((uint64_t)n + 31) / 32;
             EEG_TRY(bw.bitmap.alloc(nwords * 4 + 4));
@@ -1526,6 +1577,10 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
     }
     EEG_HIP(hipEventRecord(e1, s));
     bw.epoch = base + j + 2;
+    if (ovq_out) {
+        for (uint32_t i = 0; i < MAXW; ++i) ovq_out[i] = 0;
+        if (lm == LM_D8) EEG_HIP(hipMemcpyAsync(ovq_out, bw.ovq.p, MAXW * 8, hipMemcpyDeviceToHost, s));
+    }
     // answers: query-major, original vertex order
     const uint64_t outn = (uint64_t)nsrc * n;
     if (out) {
@@ -1561,28 +1616,112 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
 
 }  // namespace
 
+namespace {
+
+// A pass on lm labels; a 16-bit pass that overflows is answered again on 32-bit labels.
+eeg_status pass_or_wider(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                         void* out, bool out_device, eeg_stats* st, int lm, uint64_t ovq[MAXW]) {
+    bool overflow = false;
+    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, lm, &overflow, ovq));
+    if (!overflow) return EEG_OK;
+    overflow = false;
+    return batch_pass(g, algo, sources, nsrc, o, out, out_device, st, LM_U32, &overflow, ovq);
+}
+
+}  // namespace
+
 eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
                      void* out, bool out_device, eeg_stats* st) {
     const auto t0 = std::chrono::steady_clock::now();
+    BatchWork& bw = batch_work(g);
     int lm = LM_U32;
     if ((algo == EEG_SSSP || algo == EEG_SSWP) && g->weighted && nsrc > PLANE_DENSE_FROM && nsrc <= 64 * MAXW) {
         if (o.label_bits != 0 && o.label_bits != 32)
             return fail(EEG_ERR_INVALID_ARG, "label_bits must be 0 or 32 (got %u)", o.label_bits);
         if (o.label_bits == 0) {
+            uint32_t wmax = 0;
+            EEG_TRY(graph_wmax(g, bw, &wmax));
             if (algo == EEG_SSSP) {
-                lm = LM_D16;
-            } else {
-                uint32_t wmax = 0;
-                EEG_TRY(graph_wmax(g, batch_work(g), &wmax));
-                if (wmax <= 254) lm = LM_W8;
+                // 8-bit distances while the weights leave room for a few hops under 255; a
+                // graph whose 8-bit passes flagged many queries goes straight to 16 bits
+                lm = (wmax <= 127 && !bw.d8_off) ? LM_D8 : LM_D16;
+            } else if (wmax <= 254) {
+                lm = LM_W8;
             }
         }
     }
-    bool overflow = false;
-    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, lm, &overflow));
-    if (!overflow) return EEG_OK;
-    overflow = false;
-    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, LM_U32, &overflow));
+    uint64_t ovq[MAXW];
+    EEG_TRY(pass_or_wider(g, algo, sources, nsrc, o, out, out_device, st, lm, ovq));
+    if (lm == LM_D8) {
+        // queries whose distances outgrew 8 bits: answered again on 16-bit planes (32-bit when
+        // they are few or outgrow those too), their rows copied over the 8-bit answers
+        std::vector<uint32_t> qs, srcs;
+        for (uint32_t q = 0; q < nsrc; ++q)
+            if ((ovq[q / 64] >> (q % 64)) & 1ull) {
+                qs.push_back(q);
+                srcs.push_back(sources[q]);
+            }
+        if (st) st->narrow_reruns = (uint32_t)qs.size();
+        if (4 * qs.size() > nsrc) bw.d8_off = true;
+        if (!qs.empty()) {
+            const uint32_t nr = (uint32_t)qs.size(), n = g->n;
+            const uint64_t row = (uint64_t)n * 4;
+            DevMem dtmp;
+            std::vector<uint32_t> htmp;
+            void* tmp = nullptr;
+            if (out && out_device) {
+                EEG_TRY(dtmp.alloc(nr * row + 4));
+                tmp = dtmp.p;
+            } else if (out) {
+                htmp.resize((size_t)nr * n);
+                tmp = htmp.data();
+            }
+            eeg_stats s2;
+            memset(&s2, 0, sizeof(s2));
+            if (nr <= 8) {
+                // a few: single-query traversals (sync push on sorted queues), each about a
+                // third of a one-source pass
+                eeg_opts o1;
+                memset(&o1, 0, sizeof(o1));
+                o1.variant = EEG_SYNC_PUSH_DD;
+                o1.frontier = EEG_FRONTIER_SORTED;
+                o1.K = o.K;
+                DevMem d1;
+                if (out && !out_device) EEG_TRY(d1.alloc(row + 4));
+                for (uint32_t i = 0; i < nr; ++i) {
+                    eeg_stats s1;
+                    memset(&s1, 0, sizeof(s1));
+                    EEG_TRY(run_traversal(g, algo, g->h_perm[srcs[i]], o1, &s1));
+                    s2.ms_traversal += s1.ms_traversal;
+                    s2.launches += s1.launches;
+                    if (!out) continue;
+                    void* dst = out_device ? (void*)((uint8_t*)out + (uint64_t)qs[i] * row) : d1.p;
+                    EEG_TRY(unpermute(g, g->work.lab.p, dst, g->stream));
+                    if (!out_device)
+                        EEG_HIP(hipMemcpyAsync((uint8_t*)out + (uint64_t)qs[i] * row, d1.p, row, hipMemcpyDeviceToHost,
+                                               g->stream));
+                    EEG_HIP(hipStreamSynchronize(g->stream));
+                }
+                out = nullptr;  // rows written
+            } else {
+                const int lm2 = nr > PLANE_DENSE_FROM ? LM_D16 : LM_U32;
+                EEG_TRY(pass_or_wider(g, algo, srcs.data(), nr, o, tmp, out_device, &s2, lm2, nullptr));
+            }
+            if (out) {
+                for (uint32_t i = 0; i < nr; ++i) {
+                    uint8_t* dst = (uint8_t*)out + (uint64_t)qs[i] * row;
+                    const uint8_t* src = (const uint8_t*)tmp + (uint64_t)i * row;
+                    if (out_device) EEG_HIP(hipMemcpyAsync(dst, src, row, hipMemcpyDeviceToDevice, g->stream));
+                    else memcpy(dst, src, row);
+                }
+                if (out_device) EEG_HIP(hipStreamSynchronize(g->stream));
+            }
+            if (st) {
+                st->ms_traversal += s2.ms_traversal;
+                st->launches += s2.launches;
+            }
+        }
+    }
     if (st) st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return EEG_OK;
 }

@@ -70,6 +70,7 @@ struct PlaneArgs {
     uint32_t* bitmap;     // dense passes of eeg_run_batch: improved vertices set their bit and
                           // k_compact_frontier builds the next queue in vertex order (null: the
                           // push appends to the queue itself)
+    uint64_t* ovq;        // 8-bit distance passes: W words, queries to answer again wider
 };
 
 // One BFS level: push then finalize (launch indices 2*level-1 and 2*level).

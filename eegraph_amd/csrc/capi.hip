@@ -384,6 +384,7 @@ extern "C" eeg_status eeg_run_batch(eeg_graph* g, uint32_t algo, const uint32_t*
         st->launches += ps.launches;
         st->ms_traversal += ps.ms_traversal;
         st->label_bits = std::max(st->label_bits, ps.label_bits);
+        st->narrow_reruns += ps.narrow_reruns;
     }
     if (energy && eeg_energy_read(g->device, &e1, nullptr) == EEG_OK) st->energy_j = (e1 - e0) * 1e-6;
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -105,8 +105,10 @@ typedef struct eeg_opts {
     uint32_t energy;           /* read the amd-smi energy counter before and after the run into
                                   stats->energy_j (two counter reads, ~0.25 ms each; 0 = off) */
     uint32_t label_bits;       /* batched SSSP/SSWP (eeg_run_batch, more than 16 sources): label
-                                  width of the planes.  0 = the narrowest that holds the answers:
-                                  8-bit widest-path labels when every weight is <= 254, 16-bit
+                                  width of the planes.  0 = narrow labels: 8-bit widest-path
+                                  labels when every weight is <= 254; 8-bit distances when every
+                                  weight is <= 127 (queries with a distance past 254 are answered
+                                  again on 16-bit planes, see stats->narrow_reruns), else 16-bit
                                   distances (a pass whose distances reach 65535 is rerun with
                                   32-bit labels); 32 = 32-bit labels.  Answers are the same. */
     uint32_t reserved[5];
@@ -132,7 +134,9 @@ typedef struct eeg_stats {
     uint32_t label_bits;        /* batched SSSP/SSWP: label width of the passes that answered
                                    (8, 16 or 32, the widest over passes of 256 sources; 0 for
                                    other runs) */
-    uint32_t reserved[6];
+    uint32_t narrow_reruns;     /* batched SSSP on 8-bit distances: queries answered again with
+                                   wider labels because some distance passed 254 */
+    uint32_t reserved[5];
 } eeg_stats;
 
 typedef struct eeg_graph_info {

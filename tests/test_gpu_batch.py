@@ -151,12 +151,13 @@ def test_batched_ppr_planes_against_single_and_fp64(rmat13, nsrc):
             assert np.abs(lab[q].astype(np.float64) - single.astype(np.float64)).sum() < 1e-6
 
 
-@pytest.mark.parametrize("alg,bits", [("sssp", 16), ("sswp", 8)])
+@pytest.mark.parametrize("alg,bits", [("sssp", 8), ("sswp", 8)])
 @pytest.mark.parametrize("nsrc", [17, 37, 256])
 @pytest.mark.parametrize("K", [4, 8, 16, 32])
 def test_batch_narrow_labels_equal_32_bit(rmat13, alg, bits, nsrc, K):
-    """More than 16 sources take narrow label planes (16-bit distances; 8-bit widest-path
-    labels, every weight here <= 64); their answers are the 32-bit planes' bit for bit."""
+    """More than 16 sources take narrow label planes (every weight here <= 64: 8-bit distances,
+    queries past 254 answered again wider; 8-bit widest-path labels); their answers are the
+    32-bit planes' bit for bit."""
     n, s, d, w, g = rmat13
     srcs = np.random.default_rng(nsrc + K).integers(0, n, nsrc).astype(np.uint32)
     srcs[0] = int(np.argmax(np.bincount(s, minlength=n)))
@@ -216,6 +217,37 @@ def test_batch_widest_labels_need_weights_below_255(wmax, bits):
             ref = O.reference_labels(O.SSWP, n, s, d, w, int(x))
             assert np.array_equal(out[q], ref), q
             assert out[q][x] == ref[x]  # the source keeps INF
+    finally:
+        g.close()
+
+
+def _heavy_chain(step):
+    """A path 0 -> 1 -> ... -> 9 with every weight `step`, a side edge 10 -> 11 and a shortcut
+    0 -> 5 of weight 1."""
+    n = 12
+    s = np.array(list(range(9)) + [10, 0], np.uint32)
+    d = np.array(list(range(1, 10)) + [11, 5], np.uint32)
+    w = np.array([step] * 9 + [3, 1], np.uint32)
+    return n, s, d, w
+
+
+@pytest.mark.parametrize("step", [31, 50, 100, 127])
+def test_batch_8_bit_distances_rerun_the_queries_past_254(step):
+    """Weights <= 127 take 8-bit distances; a query with a finite distance past 254 is flagged
+    and answered again on wider planes, the others keep their 8-bit answers.  From source x the
+    farthest vertex is 9, at (9 - x) * step, or (4 * step + 1) through the shortcut from 0."""
+    n, s, d, w = _heavy_chain(step)
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    try:
+        srcs = np.array([0] * 6 + [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 0, 3, 7], np.uint32)
+        out, st = g.run_batch("sssp", srcs)
+        assert st["label_bits"] == 8
+        far = [(4 * step + 1) if x == 0 else ((9 - x) * step if x <= 9 else 0) for x in srcs]
+        assert st["narrow_reruns"] == sum(1 for f in far if f > 254)
+        for q, x in enumerate(srcs):
+            assert np.array_equal(out[q], O.reference_labels(O.SSSP, n, s, d, w, int(x))), q
+        wide, _ = g.run_batch("sssp", srcs, label_bits=32)
+        assert np.array_equal(out, wide)
     finally:
         g.close()
 
