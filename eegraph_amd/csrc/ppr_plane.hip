@@ -214,10 +214,23 @@ __global__ __launch_bounds__(WG) void k_pps_fix(PlaneArgsP a, double* res_part) 
         const uint32_t v = a.ent[i];
         const float2 xo = old_rank<B>(a, v, q, src, fresh);
         const uint32_t od = a.outdeg[v];
+        // The partials are added one after another in segment order; a hub has thousands
+        // (RMAT-24: 5794), so the loads of the next 8 are issued before the current 8 are
+        // added, and the chain waits on the adds rather than on a load per segment.
         Comp acc{0.0f, 0.0f};
-        for (uint32_t k = 0; k < ns; ++k) {
-            const float2 p = a.spill[(uint64_t)(mb + k) * B + q];
-            acc = comp_add(acc, Comp{p.x, p.y});
+        const float2* sp = a.spill + (uint64_t)mb * B + q;
+        float2 cur[8], nxt[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) cur[r] = ((uint32_t)r < ns) ? sp[(uint64_t)r * B] : make_float2(0.0f, 0.0f);
+        for (uint32_t k = 0; k < ns; k += 8) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                nxt[r] = (k + 8 + r < ns) ? sp[(uint64_t)(k + 8 + r) * B] : make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (k + r < ns) acc = comp_add(acc, Comp{cur[r].x, cur[r].y});
+#pragma unroll
+            for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
         }
         finish_col<B>(a, v, q, src, xo, od, acc, res);
     }
