@@ -97,6 +97,10 @@ struct Tigr {
 struct PprSeg {
     uint32_t nent = 0, nseg = 0, nmulti = 0, nspill = 0;
     DevMem ent, sscan, sdesc, mlist, mbase;
+    // hubs (more than PPR_HUB_SEGS segments): {first spill slot, segments, first chunk, 0} each,
+    // ascending by entry; nhub = 0 when there are none or too many to list
+    DevMem hubs;
+    uint32_t nhub = 0, nhubch = 0;
     bool built = false;
 };
 

@@ -10,7 +10,9 @@
 // group of B lanes (one per column) sums one segment in edge order with compensated FP32
 // (TwoSum), WAVE/B segments per wavefront.  A vertex with one segment is finished at once;
 // the partial sums of longer vertices go to spill slots and a second pass adds them in segment
-// order.  The order of every sum is fixed, so results do not depend on the schedule.
+// order; a hub's (more than PPR_HUB_SEGS segments) are first added in chunks of PPR_HUB_SEGS
+// segments by separate groups, then the chunk sums in order.  The order of every sum is fixed,
+// so results do not depend on the schedule.
 // Each column's L1 change is accumulated in FP64 per lane and reduced in a fixed order.
 //
 // Columns are independent: a column may start (fresh: x0 = e_s, nothing read from the planes)
@@ -71,6 +73,18 @@ struct MultiStore {
     }
 };
 
+// Hubs among the multi-segment entries: {entry, first spill slot, segments} appended to out
+// (at most cap; *count may pass cap, which the host reads as "too many").
+__global__ void k_find_hubs(const uint32_t* __restrict__ mlist, const uint32_t* __restrict__ sscan,
+                            const uint32_t* __restrict__ mbase, uint32_t nmulti, uint32_t* __restrict__ count,
+                            uint32_t cap, uint4* __restrict__ out) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nmulti; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = mlist[j], ns = sscan[i + 1] - sscan[i];
+        if (ns <= PPR_HUB_SEGS) continue;
+        const uint32_t pos = atomicAdd(count, 1u);
+        if (pos < cap) out[pos] = make_uint4(i, mbase[i], ns, 0u);
+    }
+}
 __global__ void k_put(uint32_t* p, uint32_t v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
@@ -113,6 +127,7 @@ struct PlaneArgsP {
     double* res_part;
     float damping;
     PprCols cols;
+    uint32_t hub_chunks;     // 1: hubs' spill slots hold chunk sums every PPR_HUB_SEGS slots
 };
 
 // Old rank of entry vertex v in column q (x0 = e_src for a fresh column, nothing read).
@@ -197,6 +212,54 @@ __global__ __launch_bounds__(WG) void k_pps_pull(PlaneArgsP a) {
     store_residual<B>(res, a.res_part);
 }
 
+// cnt compensated partials at p, p + stride, ... added in order.  A hub has thousands of
+// them (RMAT-24: 5794 segments), so the loads of the next 8 are issued before the current 8
+// are added: the chain waits on the adds rather than on a load per partial.
+template <int B>
+__device__ __forceinline__ Comp sum_slots(const float2* __restrict__ p, uint32_t cnt, uint64_t stride) {
+    Comp acc{0.0f, 0.0f};
+    float2 cur[8], nxt[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) cur[r] = ((uint32_t)r < cnt) ? p[(uint64_t)r * stride] : make_float2(0.0f, 0.0f);
+    for (uint32_t k = 0; k < cnt; k += 8) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            nxt[r] = (k + 8 + r < cnt) ? p[(uint64_t)(k + 8 + r) * stride] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (k + r < cnt) acc = comp_add(acc, Comp{cur[r].x, cur[r].y});
+#pragma unroll
+        for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
+    }
+    return acc;
+}
+
+// Hub chunks: chunk c of hub h (PPR_HUB_SEGS segments from its spill slot base + c *
+// PPR_HUB_SEGS) is added in segment order and its sum replaces the chunk's first slot (read
+// before the write by the same lanes).  hubs[h] = {entry, first slot, segments, first chunk}.
+template <int B>
+__global__ __launch_bounds__(WG) void k_pps_hub_chunks(float2* __restrict__ spill, const uint4* __restrict__ hubs,
+                                                       uint32_t nhub, uint32_t nch) {
+    constexpr int G = WAVE / B;
+    const uint32_t lane = lane_id(), q = lane % B, grp = lane / B;
+    const uint64_t nw = (uint64_t)gridDim.x * (WG / WAVE);
+    for (uint64_t w = (uint64_t)blockIdx.x * (WG / WAVE) + threadIdx.x / WAVE; w * G < nch; w += nw) {
+        const uint64_t t = w * G + grp;
+        if (t >= nch) continue;
+        uint32_t lo = 0, hi = nhub;  // last hub whose first chunk is <= t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (hubs[mid].w <= t) lo = mid; else hi = mid;
+        }
+        const uint4 h = hubs[lo];
+        const uint32_t c = (uint32_t)t - h.w;
+        const uint32_t first = c * PPR_HUB_SEGS, cnt = min(PPR_HUB_SEGS, h.z - first);
+        float2* p = spill + (uint64_t)(h.y + first) * B + q;
+        const Comp acc = sum_slots<B>(p, cnt, (uint64_t)B);
+        *p = make_float2(acc.s, acc.e);
+    }
+}
+
 // Entries with several segments: their partial sums added in segment order.
 template <int B>
 __global__ __launch_bounds__(WG) void k_pps_fix(PlaneArgsP a, double* res_part) {
@@ -214,24 +277,11 @@ __global__ __launch_bounds__(WG) void k_pps_fix(PlaneArgsP a, double* res_part) 
         const uint32_t v = a.ent[i];
         const float2 xo = old_rank<B>(a, v, q, src, fresh);
         const uint32_t od = a.outdeg[v];
-        // The partials are added one after another in segment order; a hub has thousands
-        // (RMAT-24: 5794), so the loads of the next 8 are issued before the current 8 are
-        // added, and the chain waits on the adds rather than on a load per segment.
-        Comp acc{0.0f, 0.0f};
-        const float2* sp = a.spill + (uint64_t)mb * B + q;
-        float2 cur[8], nxt[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) cur[r] = ((uint32_t)r < ns) ? sp[(uint64_t)r * B] : make_float2(0.0f, 0.0f);
-        for (uint32_t k = 0; k < ns; k += 8) {
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                nxt[r] = (k + 8 + r < ns) ? sp[(uint64_t)(k + 8 + r) * B] : make_float2(0.0f, 0.0f);
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (k + r < ns) acc = comp_add(acc, Comp{cur[r].x, cur[r].y});
-#pragma unroll
-            for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
-        }
+        // a hub's chunk sums sit every PPR_HUB_SEGS slots (k_pps_hub_chunks)
+        const bool hub = a.hub_chunks && ns > PPR_HUB_SEGS;
+        const uint32_t cnt = hub ? (ns + PPR_HUB_SEGS - 1) / PPR_HUB_SEGS : ns;
+        const uint64_t stride = hub ? (uint64_t)PPR_HUB_SEGS * B : (uint64_t)B;
+        const Comp acc = sum_slots<B>(a.spill + (uint64_t)mb * B + q, cnt, stride);
         finish_col<B>(a, v, q, src, xo, od, acc, res);
     }
     store_residual<B>(res, res_part);
@@ -314,6 +364,37 @@ eeg_status ensure_ppr_seg(eeg_graph* g) {
     EEG_HIP(hipStreamSynchronize(st));
     t.nmulti = (uint32_t)(tot[1] & 0xffffffffull);
     t.nspill = (uint32_t)(tot[1] >> 32);
+    // hubs, for the chunked addition of their partial sums
+    t.nhub = t.nhubch = 0;
+    if (t.nmulti) {
+        const uint32_t cap = 1u << 16;
+        EEG_TRY(t.hubs.alloc((size_t)cap * 16));
+        EEG_HIP(hipMemsetAsync(total.p, 0, 4, st));
+        k_find_hubs<<<grid_of(t.nmulti, 8192), WG, 0, st>>>(t.mlist.as<uint32_t>(), t.sscan.as<uint32_t>(),
+                                                             t.mbase.as<uint32_t>(), t.nmulti, total.as<uint32_t>(),
+                                                             cap, t.hubs.as<uint4>());
+        EEG_HIP(hipGetLastError());
+        uint32_t nh = 0;
+        EEG_HIP(hipMemcpyAsync(&nh, total.p, 4, hipMemcpyDeviceToHost, st));
+        EEG_HIP(hipStreamSynchronize(st));
+        if (nh > 0 && nh <= cap) {
+            std::vector<uint4> h(nh);
+            EEG_HIP(hipMemcpyAsync(h.data(), t.hubs.p, (size_t)nh * 16, hipMemcpyDeviceToHost, st));
+            EEG_HIP(hipStreamSynchronize(st));
+            std::sort(h.begin(), h.end(), [](const uint4& a, const uint4& b) { return a.x < b.x; });
+            uint64_t ch = 0;
+            for (auto& x : h) {
+                x.w = (uint32_t)ch;
+                ch += (x.z + PPR_HUB_SEGS - 1) / PPR_HUB_SEGS;
+            }
+            if (ch < 0xffffffffull) {
+                EEG_HIP(hipMemcpyAsync(t.hubs.p, h.data(), (size_t)nh * 16, hipMemcpyHostToDevice, st));
+                EEG_HIP(hipStreamSynchronize(st));
+                t.nhub = nh;
+                t.nhubch = (uint32_t)ch;
+            }
+        }
+    }
     EEG_TRY(t.sdesc.alloc((size_t)t.nseg * 16 + 16));
     if (t.nent)
         k_seg_desc<<<grid_of(t.nent, 8192), WG, 0, st>>>(row, t.ent.as<uint32_t>(), t.sscan.as<uint32_t>(),
@@ -373,17 +454,22 @@ eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipS
     a.res_part = res_part.as<double>();
     a.damping = damping;
     a.cols = cols;
+    a.hub_chunks = t.nhub ? 1u : 0u;
     double* fix_part = res_part.as<double>() + (size_t)grid_pull * B;
+    const unsigned grid_hub = grid_of((uint64_t)t.nhubch * B, 4096);
     if (B == 8) {
         k_pps_pull<8><<<grid_pull, WG, 0, s>>>(a);
+        if (t.nhub) k_pps_hub_chunks<8><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
         k_pps_fix<8><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<8><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
     } else if (B == 16) {
         k_pps_pull<16><<<grid_pull, WG, 0, s>>>(a);
+        if (t.nhub) k_pps_hub_chunks<16><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
         k_pps_fix<16><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<16><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
     } else {
         k_pps_pull<32><<<grid_pull, WG, 0, s>>>(a);
+        if (t.nhub) k_pps_hub_chunks<32><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
         k_pps_fix<32><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<32><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
     }

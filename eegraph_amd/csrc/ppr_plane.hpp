@@ -8,6 +8,8 @@
 namespace eeg {
 
 constexpr uint32_t PPR_SEG = 64;      // in-edges per segment of the gather
+constexpr uint32_t PPR_HUB_SEGS = 64;  // entries with more segments are hubs: their partial sums
+                                      // are added in chunks of PPR_HUB_SEGS segments first
 constexpr uint32_t PPR_IDLE = 0xffffffffu;
 
 // Columns of one iteration: restart vertex (internal id, PPR_IDLE for an unused column) and the

@@ -151,6 +151,33 @@ def test_batched_ppr_planes_against_single_and_fp64(rmat13, nsrc):
             assert np.abs(lab[q].astype(np.float64) - single.astype(np.float64)).sum() < 1e-6
 
 
+@pytest.mark.parametrize("nsrc", [3, 40])
+def test_batched_ppr_hub_chunks(nsrc):
+    """Vertices with more than 64 segments of 64 in-edges (in-degree > 4096) add their partial
+    sums in chunks of 64 segments first: two hubs (in-degrees 9000 and 4500, just past one
+    chunk) and one at exactly 64 segments (4096, no chunks) against the FP64 oracle, with
+    restart vertices on and next to the hubs.  Run to an L1 change below 1e-10, so the distance
+    left to the fixed point (about 5.7x the last change) stays far below the tolerance; with
+    the default 1e-7 a restart on vertex 7 stops ~1.1e-7 short there."""
+    n = 10000
+    rng = np.random.default_rng(11)
+    s = np.concatenate([np.arange(1, 9001), np.arange(5000, 9500), np.arange(2000, 6096),
+                        rng.integers(0, n, 30000)]).astype(np.uint32)
+    d = np.concatenate([np.zeros(9000), np.full(4500, 7), np.full(4096, 11),
+                        rng.integers(12, n, 30000)]).astype(np.uint32)
+    g = eg.Graph.from_edges(s, d, np.ones_like(s), n=n)
+    try:
+        row, col, _ = O.csr(n, s, d)
+        srcs = np.array([0, 7, 11, 5, 9000, 9999, 1234][:nsrc] + list(rng.integers(0, n, max(0, nsrc - 7))), np.uint32)
+        lab, st = g.run_batch("ppr", srcs, epsilon=1e-10)
+        for q, x in enumerate(srcs):
+            ref, _, _ = O.pagerank(n, row, col, 0.85, 1e-13, personalised=True, source=int(x))
+            err = np.abs(lab[q].astype(np.float64) - ref)
+            assert err.sum() < 1e-6 and err.max() < 1e-7, (q, int(x), err.sum(), err.max())
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("alg,bits", [("sssp", 8), ("sswp", 8)])
 @pytest.mark.parametrize("nsrc", [17, 37, 256])
 @pytest.mark.parametrize("K", [4, 8, 16, 32])
