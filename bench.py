@@ -66,9 +66,10 @@ def parse():
                    help="queries at the highest offered load (the saturated throughput point; long enough "
                         "that the last queries' latency does not set the rate: 600 measured 2.3-2.45x, "
                         "1500 2.6x on the same server)")
-    p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp"),
+    p.add_argument("--batched-only", choices=("bfs", "sssp", "sswp", "ppr"),
                    help="only run two multi-source passes (the second timed) of this algorithm on the "
-                        "batched section's graph and sources, for profiling the batched kernels")
+                        "batched section's graph and sources (ppr: the first 64, as in the batched "
+                        "section), for profiling the batched kernels")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 512 MiB device-to-device copy kernel three times "
                         "(scale reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -446,14 +447,18 @@ def batched_only(eg, a):
     g = eg.Graph.rmat(a.batch_scale, 16, seed=4)
     n = g.info()["n"]
     srcs = np.random.default_rng(4).integers(0, n, 256).astype(np.uint32)
+    if a.batched_only == "ppr":
+        srcs = srcs[:64]
     g.run_batch(a.batched_only, srcs, want_labels=False)
     t = time.perf_counter()
     _, st = g.run_batch(a.batched_only, srcs, want_labels=False)
     bat = time.perf_counter() - t
-    print(json.dumps({"batched_only": a.batched_only, "workload": f"rmat{a.batch_scale}_ef16_seed4_256_sources",
+    print(json.dumps({"batched_only": a.batched_only,
+                      "workload": f"rmat{a.batch_scale}_ef16_seed4_{len(srcs)}_sources",
                       "ms": round(bat * 1e3, 3), "ms_traversal": round(st["ms_traversal"], 3),
-                      "queries_per_s": round(256 / bat, 1), "iterations": st["iterations"],
-                      "label_bits": st["label_bits"], "narrow_reruns": st["narrow_reruns"]}), flush=True)
+                      "queries_per_s": round(len(srcs) / bat, 1), "iterations": st["iterations"],
+                      "plane_iterations": st["launches"], "label_bits": st["label_bits"],
+                      "narrow_reruns": st["narrow_reruns"]}), flush=True)
     g.close()
 
 
