@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "bytes8.hpp"
 #include "frontier.hpp"
 
 namespace eeg {
@@ -353,45 +354,9 @@ constexpr uint32_t PLANE_ERR_NARROW = 64u;
 
 template <int LM>
 struct Packed;
-// Bytewise helpers for four 8-bit labels per word: pairs of bytes in 16-bit halves with packed
-// min/max, no branch per label.
-typedef unsigned short eeg_us2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(eeg_us2, a),
-                                                                  __builtin_bit_cast(eeg_us2, b)));
-}
-__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(eeg_us2, a),
-                                                                  __builtin_bit_cast(eeg_us2, b)));
-}
-// bit 7 of each byte -> bits 0..3
-__device__ __forceinline__ uint32_t pack4(uint32_t hb) { return (((hb >> 7) & 0x01010101u) * 0x01020408u) >> 24; }
-// bit 7 set in exactly the bytes of t that are zero
-__device__ __forceinline__ uint32_t zero_bytes(uint32_t t) {
-    return ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t | 0x7f7f7f7fu);
-}
-// 0xff in the bytes whose bit of nib is set
-__device__ __forceinline__ uint32_t byte_sel(uint32_t nib) {
-    return ((nib & 1u) | ((nib & 2u) << 7) | ((nib & 4u) << 14) | ((nib & 8u) << 21)) * 0xffu;
-}
-// Widest-path labels, four per word.
+// Widest-path labels, four per word (bytes8.hpp).
 template <>
-struct Packed<LM_W8> {
-    static constexpr uint32_t QPL = 4, BITS = 8, FM = 0xffu;  // labels per 32-bit lane word
-    // candidates of the queries in `nib` over an edge of weight w (<= 254): min(label, w); 0
-    // (no candidate) for the others
-    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t&) {
-        const uint32_t l = lv & byte_sel(nib), ww = w * 0x00010001u;
-        return pk_min_u16(l & 0x00ff00ffu, ww) | (pk_min_u16((l >> 8) & 0x00ff00ffu, ww) << 8);
-    }
-    __device__ static uint32_t merge(uint32_t cur, uint32_t c) {  // per-label max
-        return pk_max_u16(cur & 0x00ff00ffu, c & 0x00ff00ffu) |
-               (pk_max_u16((cur >> 8) & 0x00ff00ffu, (c >> 8) & 0x00ff00ffu) << 8);
-    }
-    __device__ static uint32_t improved(uint32_t cur, uint32_t c) {
-        return pack4(~zero_bytes(merge(cur, c) ^ cur) & 0x80808080u);
-    }
-};
+struct Packed<LM_W8> : Bytes8Widest {};
 // Distances of BITS bits per label, all ones = INF.  cand() sets bit i of `ov` for a query
 // whose finite candidate does not fit.
 template <uint32_t Q, uint32_t NB>
@@ -431,32 +396,9 @@ struct PackedDist {
 };
 template <>
 struct Packed<LM_D16> : PackedDist<2, 16> {};
-// 8-bit distances with bytewise arithmetic on the whole word, for weights <= 254: the same
-// results as PackedDist<4, 8> without a branch per label (those branches were most of the
-// pass's scalar instructions).  Pairs of bytes are added in 16-bit halves, so a sum up to
-// 255 + 254 keeps its carry in bit 8, which saturates the byte to 255 (INF).
+// 8-bit distances, four per word (bytes8.hpp).
 template <>
-struct Packed<LM_D8> {
-    static constexpr uint32_t QPL = 4, BITS = 8, FM = 0xffu;
-    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
-        const uint32_t l = lv | ~byte_sel(nib);  // labels not asked for count as INF
-        const uint32_t ww = w * 0x00010001u;
-        uint32_t se = (l & 0x00ff00ffu) + ww, so = ((l >> 8) & 0x00ff00ffu) + ww;
-        se = (se | (((se >> 8) & 0x00010001u) * 0xffu)) & 0x00ff00ffu;
-        so = (so | (((so >> 8) & 0x00010001u) * 0xffu)) & 0x00ff00ffu;
-        const uint32_t c = se | (so << 8);
-        ov |= pack4(zero_bytes(~c) & ~zero_bytes(~l));  // finite labels whose sum does not fit
-        return c;
-    }
-    __device__ static uint32_t merge(uint32_t cur, uint32_t c) {  // per-label min
-        return pk_min_u16(cur & 0x00ff00ffu, c & 0x00ff00ffu) |
-               (pk_min_u16((cur >> 8) & 0x00ff00ffu, (c >> 8) & 0x00ff00ffu) << 8);
-    }
-    __device__ static uint32_t improved(uint32_t cur, uint32_t c) {
-        return pack4(~zero_bytes(merge(cur, c) ^ cur) & 0x80808080u);
-    }
-    __device__ static uint32_t unreached(uint32_t cur) { return pack4(zero_bytes(~cur)); }
-};
+struct Packed<LM_D8> : Bytes8Dist {};
 
 // Dense variant: one wavefront per virtual node.  LM_U32: every pending 64-query word handled
 // with lane = query, so the label reads and atomics of a wavefront are contiguous 256-byte
