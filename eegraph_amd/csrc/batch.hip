@@ -519,15 +519,15 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                     uint32_t ecur[K];
 #pragma unroll
                     for (int e = 0; e < K; ++e) {  // every destination row read before any update
-                        const uint32_t u = __shfl(my_u, e, WAVE);
+                        const uint32_t u = __builtin_amdgcn_readlane(my_u, e);
                         ecur[e] = (nib && (uint32_t)e < ne) ? plane[(uint64_t)u * nl + x] : 0u;
                     }
                     uint32_t imp[K];
 #pragma unroll
                     for (int e = 0; e < K; ++e) {
                         imp[e] = 0;
-                        const uint32_t u = __shfl(my_u, e, WAVE);
-                        const uint32_t wt = __shfl(my_w, e, WAVE);
+                        const uint32_t u = __builtin_amdgcn_readlane(my_u, e);
+                        const uint32_t wt = __builtin_amdgcn_readlane(my_w, e);
                         if (nib && (uint32_t)e < ne) {
                             uint32_t ov = 0;
                             const uint32_t cd = P::cand(lv, wt, nib, ov);
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                     }
 #pragma unroll
                     for (int e = 0; e < K; ++e) {
-                        const uint32_t u = __shfl(my_u, e, WAVE);
+                        const uint32_t u = __builtin_amdgcn_readlane(my_u, e);
                         if (!__ballot(imp[e] != 0)) continue;
                         // only vertices with out-edges carry changes and are queued (each
                         // queued entry owns >= 1 virtual node); a sink's label is final
