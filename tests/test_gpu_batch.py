@@ -279,6 +279,24 @@ def test_batch_8_bit_distances_rerun_the_queries_past_254(step):
         g.close()
 
 
+def test_batch_8_bit_reruns_on_16_bit_planes():
+    """More than 16 flagged queries are answered again together on 16-bit planes: 40 sources on
+    the heavy chain (weight 100), 30 of them with a distance past 254."""
+    n, s, d, w = _heavy_chain(100)
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    try:
+        srcs = np.array([0] * 14 + [1, 2, 3, 4, 5, 6] * 2 + [7, 8, 9, 10, 11] * 2 + [1, 2, 3, 4], np.uint32)
+        assert len(srcs) == 40
+        far = [(4 * 100 + 1) if x == 0 else ((9 - x) * 100 if x <= 9 else 0) for x in srcs]
+        out, st = g.run_batch("sssp", srcs)
+        assert st["label_bits"] == 8
+        assert st["narrow_reruns"] == sum(1 for f in far if f > 254) == 30
+        for q, x in enumerate(srcs):
+            assert np.array_equal(out[q], O.reference_labels(O.SSSP, n, s, d, w, int(x))), q
+    finally:
+        g.close()
+
+
 def test_batch_label_bits_must_be_0_or_32(rmat13):
     n, s, d, w, g = rmat13
     with pytest.raises(eg.EEGError):
