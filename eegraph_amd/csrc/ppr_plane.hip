@@ -128,12 +128,14 @@ struct PlaneArgsP {
     float damping;
     PprCols cols;
     uint32_t hub_chunks;     // 1: hubs' spill slots hold chunk sums every PPR_HUB_SEGS slots
+    uint32_t S;              // row stride of the planes and spill slots (columns allocated);
+                             // a pass of B <= S columns reads and writes the first B of a row
 };
 
 // Old rank of entry vertex v in column q (x0 = e_src for a fresh column, nothing read).
 template <int B>
 __device__ __forceinline__ float2 old_rank(const PlaneArgsP& a, uint32_t v, uint32_t q, uint32_t src, bool fresh) {
-    return fresh ? make_float2(v == src ? 1.0f : 0.0f, 0.0f) : a.x_old[(uint64_t)v * B + q];
+    return fresh ? make_float2(v == src ? 1.0f : 0.0f, 0.0f) : a.x_old[(uint64_t)v * a.S + q];
 }
 
 // New rank of entry vertex v in column q from its gathered sum; adds the L1 change to res.
@@ -142,7 +144,7 @@ __device__ __forceinline__ void finish_col(const PlaneArgsP& a, uint32_t v, uint
                                            uint32_t od, Comp sum, double& res) {
     float hi, lo;
     rank_pair(v == src ? 1.0f - a.damping : 0.0f, a.damping, sum, &hi, &lo);
-    const uint64_t o = (uint64_t)v * B + q;
+    const uint64_t o = (uint64_t)v * a.S + q;
     a.x_new[o] = make_float2(hi, lo);
     a.c_new[o] = od ? (hi + lo) / (float)od : 0.0f;
     res += fabs(((double)hi + (double)lo) - ((double)xo.x + (double)xo.y));
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(WG) void k_pps_pull(PlaneArgsP a) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (fresh) cv[k] = (u[k] == src) ? c0 : 0.0f;
-                else cv[k] = (u[k] != PPR_IDLE) ? a.c_old[(uint64_t)u[k] * B + q] : 0.0f;
+                else cv[k] = (u[k] != PPR_IDLE) ? a.c_old[(uint64_t)u[k] * a.S + q] : 0.0f;
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) u[k] = (b + 8 + k < len) ? a.col[e0 + b + 8 + k] : PPR_IDLE;
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(WG) void k_pps_pull(PlaneArgsP a) {
                 if (b + k < len) acc = comp_add(acc, Comp{cv[k], 0.0f});
         }
         if (single) finish_col<B>(a, v, q, src, xo, od, acc, res);
-        else a.spill[(uint64_t)dsc.w * B + q] = make_float2(acc.s, acc.e);
+        else a.spill[(uint64_t)dsc.w * a.S + q] = make_float2(acc.s, acc.e);
     }
     store_residual<B>(res, a.res_part);
 }
@@ -239,7 +241,7 @@ __device__ __forceinline__ Comp sum_slots(const float2* __restrict__ p, uint32_t
 // before the write by the same lanes).  hubs[h] = {entry, first slot, segments, first chunk}.
 template <int B>
 __global__ __launch_bounds__(WG) void k_pps_hub_chunks(float2* __restrict__ spill, const uint4* __restrict__ hubs,
-                                                       uint32_t nhub, uint32_t nch) {
+                                                       uint32_t nhub, uint32_t nch, uint32_t S) {
     constexpr int G = WAVE / B;
     const uint32_t lane = lane_id(), q = lane % B, grp = lane / B;
     const uint64_t nw = (uint64_t)gridDim.x * (WG / WAVE);
@@ -254,8 +256,8 @@ __global__ __launch_bounds__(WG) void k_pps_hub_chunks(float2* __restrict__ spil
         const uint4 h = hubs[lo];
         const uint32_t c = (uint32_t)t - h.w;
         const uint32_t first = c * PPR_HUB_SEGS, cnt = min(PPR_HUB_SEGS, h.z - first);
-        float2* p = spill + (uint64_t)(h.y + first) * B + q;
-        const Comp acc = sum_slots<B>(p, cnt, (uint64_t)B);
+        float2* p = spill + (uint64_t)(h.y + first) * S + q;
+        const Comp acc = sum_slots<B>(p, cnt, (uint64_t)S);
         *p = make_float2(acc.s, acc.e);
     }
 }
@@ -280,8 +282,8 @@ __global__ __launch_bounds__(WG) void k_pps_fix(PlaneArgsP a, double* res_part) 
         // a hub's chunk sums sit every PPR_HUB_SEGS slots (k_pps_hub_chunks)
         const bool hub = a.hub_chunks && ns > PPR_HUB_SEGS;
         const uint32_t cnt = hub ? (ns + PPR_HUB_SEGS - 1) / PPR_HUB_SEGS : ns;
-        const uint64_t stride = hub ? (uint64_t)PPR_HUB_SEGS * B : (uint64_t)B;
-        const Comp acc = sum_slots<B>(a.spill + (uint64_t)mb * B + q, cnt, stride);
+        const uint64_t stride = hub ? (uint64_t)PPR_HUB_SEGS * a.S : (uint64_t)a.S;
+        const Comp acc = sum_slots<B>(a.spill + (uint64_t)mb * a.S + q, cnt, stride);
         finish_col<B>(a, v, q, src, xo, od, acc, res);
     }
     store_residual<B>(res, res_part);
@@ -432,7 +434,11 @@ eeg_status PprPlane::init(eeg_graph* g, uint32_t Bc, float d, hipStream_t s) {
     return EEG_OK;
 }
 
-eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipStream_t s) {
+eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipStream_t s, uint32_t W) {
+    if (W == 0) W = B;
+    if ((W != 8 && W != 16 && W != 32) || W > B) return fail(EEG_ERR_INVALID_ARG, "pass width %u of %u columns", W, B);
+    for (uint32_t q = W; q < B; ++q)
+        if (cols.src[q] != PPR_IDLE) return fail(EEG_ERR_INVALID_ARG, "column %u is beyond a pass of %u", q, W);
     const PprSeg& t = g->pprseg;
     const int rb = (int)((j & 1u) ^ 1u), wb = (int)(j & 1u);
     PlaneArgsP a{};
@@ -455,26 +461,36 @@ eeg_status PprPlane::iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipS
     a.damping = damping;
     a.cols = cols;
     a.hub_chunks = t.nhub ? 1u : 0u;
-    double* fix_part = res_part.as<double>() + (size_t)grid_pull * B;
-    const unsigned grid_hub = grid_of((uint64_t)t.nhubch * B, 4096);
-    if (B == 8) {
+    a.S = B;
+    // residual partials and the reduction are W columns wide
+    double* fix_part = res_part.as<double>() + (size_t)grid_pull * W;
+    const unsigned grid_hub = grid_of((uint64_t)t.nhubch * W, 4096);
+    if (W == 8) {
         k_pps_pull<8><<<grid_pull, WG, 0, s>>>(a);
-        if (t.nhub) k_pps_hub_chunks<8><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
+        if (t.nhub) k_pps_hub_chunks<8><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch, B);
         k_pps_fix<8><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<8><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
-    } else if (B == 16) {
+    } else if (W == 16) {
         k_pps_pull<16><<<grid_pull, WG, 0, s>>>(a);
-        if (t.nhub) k_pps_hub_chunks<16><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
+        if (t.nhub) k_pps_hub_chunks<16><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch, B);
         k_pps_fix<16><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<16><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
     } else {
         k_pps_pull<32><<<grid_pull, WG, 0, s>>>(a);
-        if (t.nhub) k_pps_hub_chunks<32><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch);
+        if (t.nhub) k_pps_hub_chunks<32><<<grid_hub, WG, 0, s>>>(a.spill, t.hubs.as<uint4>(), t.nhub, t.nhubch, B);
         k_pps_fix<32><<<grid_fix, WG, 0, s>>>(a, fix_part);
         k_pps_reduce<32><<<1, WG, 0, s>>>(res_part.as<double>(), grid_pull + grid_fix, resid.as<double>());
     }
     EEG_HIP(hipGetLastError());
     return EEG_OK;
+}
+
+uint32_t PprPlane::width_for(const PprCols& cols) const {
+    uint32_t top = 0;  // one past the highest busy column
+    for (uint32_t q = 0; q < B; ++q)
+        if (cols.src[q] != PPR_IDLE) top = q + 1;
+    const uint32_t w = top <= 8 ? 8u : top <= 16 ? 16u : 32u;
+    return std::min(w, B);
 }
 
 eeg_status PprPlane::extract(eeg_graph* g, uint32_t j, uint32_t q, uint32_t src, float* out, hipStream_t s) {
@@ -533,7 +549,9 @@ eeg_status run_ppr_batch(eeg_graph* g, const uint32_t* sources, uint32_t nsrc, c
             colq[q] = next++;
             start[q] = j;
         }
-        EEG_TRY(pl.iterate(g, j, cols, s));
+        // columns are refilled lowest first, so a pass needs only the width of the busy ones
+        // (a pass of 8 or 16 columns costs about 0.3 or 0.6 of one of 32)
+        EEG_TRY(pl.iterate(g, j, cols, s, pl.width_for(cols)));
         EEG_HIP(hipMemcpyAsync(hres.data(), pl.resid.p, (size_t)B * 8, hipMemcpyDeviceToHost, s));
         EEG_HIP(hipStreamSynchronize(s));
         ++total_iters;

@@ -36,8 +36,12 @@ struct PprPlane {
     unsigned grid_pull = 0, grid_fix = 0;
     static size_t bytes_for(uint32_t n, uint32_t B, uint32_t nspill);
     eeg_status init(eeg_graph* g, uint32_t B, float damping, hipStream_t s);
-    // one Jacobi iteration for every column; resid receives each column's L1 change
-    eeg_status iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipStream_t s);
+    // one Jacobi iteration for every column; resid receives each column's L1 change.  W (8,
+    // 16 or 32, at most B; 0 = B) columns per pass: columns W.. must be idle, the pass reads
+    // and writes the first W of every row and resid[0..W-1]
+    eeg_status iterate(eeg_graph* g, uint32_t j, const PprCols& cols, hipStream_t s, uint32_t W = 0);
+    // narrowest pass width that covers every busy column of cols
+    uint32_t width_for(const PprCols& cols) const;
     // column q's ranks after iteration j into out (n floats, original vertex order)
     eeg_status extract(eeg_graph* g, uint32_t j, uint32_t q, uint32_t src, float* out, hipStream_t s);
 };

@@ -637,7 +637,8 @@ void PprEngine::loop() {
         if (active > 0) {
             PprCols cols = cur;
             cols.fresh = fresh;
-            rc = pl.iterate(g, it, cols, st);
+            // columns are taken lowest first: the pass covers only up to the highest busy one
+            rc = pl.iterate(g, it, cols, st, pl.width_for(cols));
             if (rc == EEG_OK &&
                 (hipMemcpyAsync(hres + (it % 4) * 32, pl.resid.p, (size_t)ncols * 8, hipMemcpyDeviceToHost, st) !=
                      hipSuccess ||
