@@ -92,6 +92,7 @@ class ServerCfg(C.Structure):
 
 
 SERVER_DISCARD_LABELS, SERVER_PLANE_ENGINES, SERVER_NO_BFS_ENGINE, SERVER_NO_PPR_ENGINE = 1, 2, 4, 8
+SERVER_NO_LANE_BATCH = 16
 
 
 class QueryResult(C.Structure):

@@ -262,7 +262,7 @@ class Server:
     def __init__(self, graph, label_budget_bytes=0, max_batch=0, K=0, damping=0.0,
                  ppr_epsilon=0.0, discard_labels=False, lane_wgs_per_cu=0, lanes=0,
                  plane_engines=False, bfs_engine=True, first_iteration=0, ppr_engine=True,
-                 ppr_columns=0):
+                 ppr_columns=0, lane_batch=True):
         cfg = L.ServerCfg()
         cfg.label_budget_bytes = label_budget_bytes
         cfg.max_batch = max_batch
@@ -272,7 +272,8 @@ class Server:
         cfg.flags = ((L.SERVER_DISCARD_LABELS if discard_labels else 0) |
                      (L.SERVER_PLANE_ENGINES if plane_engines else 0) |
                      (0 if bfs_engine else L.SERVER_NO_BFS_ENGINE) |
-                     (0 if ppr_engine else L.SERVER_NO_PPR_ENGINE))
+                     (0 if ppr_engine else L.SERVER_NO_PPR_ENGINE) |
+                     (0 if lane_batch else L.SERVER_NO_LANE_BATCH))
         cfg.lane_wgs_per_cu = lane_wgs_per_cu
         cfg.lanes = lanes
         cfg.first_iteration = first_iteration

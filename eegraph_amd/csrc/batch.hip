@@ -954,14 +954,14 @@ __global__ void k_plane_rotate(const uint32_t* q, const Ctl* ctl, int slot,
         atomicOr((unsigned long long*)&act[threadIdx.x], sAct[threadIdx.x]);
 }
 
-// out[q * n + u] = planes[perm[u] * B + q]: answers in query-major, original vertex order,
-// narrow labels widened (their all-ones INF to INF)
+// out[q * n + u] = planes[perm[u] * B + q0 + q]: answers of queries q0 .. q0+nq-1 in
+// query-major, original vertex order, narrow labels widened (their all-ones INF to INF)
 __global__ void k_planes_out(const uint32_t* __restrict__ planes, const uint32_t* __restrict__ perm,
-                             uint32_t n, uint32_t B, uint32_t nq, int lm, uint32_t* __restrict__ out) {
+                             uint32_t n, uint32_t B, uint32_t q0, uint32_t nq, int lm, uint32_t* __restrict__ out) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * nq;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(i / n), u = (uint32_t)(i % n);
-        const uint64_t j = (uint64_t)perm[u] * B + q;
+        const uint64_t j = (uint64_t)perm[u] * B + q0 + q;
         uint32_t x;
         if (lm == LM_W8 || lm == LM_D8) {
             x = reinterpret_cast<const uint8_t*>(planes)[j];
@@ -1141,6 +1141,7 @@ struct BatchWork {
     DevMem bitmap;          // dense plane passes: vertices improved this iteration (1 bit each)
     int64_t wmax = -1;      // largest edge weight of the graph (-1: not measured yet)
     DevMem ovq;             // 8-bit SSSP passes: queries whose distances outgrew 8 bits
+    DevMem rowbuf;          // one answer (n words) on its way to a host row
     bool d8_off = false;    // a previous 8-bit SSSP pass flagged more than a quarter of its queries
     DevMem q[2], s[2], cand, stamp, seeds, ctl;
     DevMem devit;           // replayed chunks: {iteration, stamp base}
@@ -1311,29 +1312,38 @@ static BatchWork& batch_work(eeg_graph* g) {
 namespace {
 
 // Largest edge weight of the graph, measured once per handle.
-eeg_status graph_wmax(eeg_graph* g, BatchWork& bw, uint32_t* wmax) {
+eeg_status graph_wmax(eeg_graph* g, BatchWork& bw, hipStream_t s, uint32_t* wmax) {
     if (bw.wmax < 0) {
         DevMem d;
         EEG_TRY(d.alloc(8));
         uint32_t h = 0;
-        EEG_HIP(hipMemsetAsync(d.p, 0, 4, g->stream));
+        EEG_HIP(hipMemsetAsync(d.p, 0, 4, s));
         if (g->m) {
-            k_max_u32<<<grid_n(g->m), WG, 0, g->stream>>>(g->out.wp(), g->m, d.as<uint32_t>());
+            k_max_u32<<<grid_n(g->m), WG, 0, s>>>(g->out.wp(), g->m, d.as<uint32_t>());
             EEG_HIP(hipGetLastError());
         }
-        EEG_HIP(hipMemcpyAsync(&h, d.p, 4, hipMemcpyDeviceToHost, g->stream));
-        EEG_HIP(hipStreamSynchronize(g->stream));
+        EEG_HIP(hipMemcpyAsync(&h, d.p, 4, hipMemcpyDeviceToHost, s));
+        EEG_HIP(hipStreamSynchronize(s));
         bw.wmax = h;
     }
     *wmax = (uint32_t)bw.wmax;
     return EEG_OK;
 }
 
-// One multi-source pass with labels of width lm.  A 16-bit pass whose distances outgrow its
-// labels sets *overflow and returns EEG_OK without answers; an 8-bit pass answers every query
-// and returns the queries to answer again (bit q of ovq[q / 64]) in ovq_out.
-eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
-                      void* out, bool out_device, eeg_stats* st, int lm, bool* overflow,
+// Where the answers of a pass go: one query-major block (host or device), or one host row per
+// query (null rows are skipped).  Neither: no answers are copied out.
+struct BatchOut {
+    void* out = nullptr;
+    bool device = false;
+    uint32_t* const* rows = nullptr;
+};
+
+// One multi-source pass with labels of width lm, on the state bw and stream s.  A 16-bit pass
+// whose distances outgrow its labels sets *overflow and returns EEG_OK without answers; an
+// 8-bit pass answers every query and returns the queries to answer again (bit q of
+// ovq[q / 64]) in ovq_out.
+eeg_status batch_pass(eeg_graph* g, BatchWork& bw, hipStream_t s, uint32_t algo, const uint32_t* sources,
+                      uint32_t nsrc, const eeg_opts& o, const BatchOut& bo, eeg_stats* st, int lm, bool* overflow,
                       uint64_t ovq_out[MAXW]) {
     const auto t_host0 = std::chrono::steady_clock::now();
     if (nsrc == 0 || nsrc > 64 * MAXW) return fail(EEG_ERR_INVALID_ARG, "nsrc must be 1..%u per pass", 64 * MAXW);
@@ -1348,8 +1358,6 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
         if (sources[i] >= n) return fail(EEG_ERR_INVALID_ARG, "source %u out of range", sources[i]);
     EEG_TRY(ensure_tigr(g, g->out, g->tout, K));
     const Tigr& t = g->tout[k_index(K)];
-    hipStream_t s = g->stream;
-    BatchWork& bw = batch_work(g);
     const uint32_t W = (nsrc + 63) / 64, B = W * 64;
     const uint64_t nw = (uint64_t)n * W, nb = (uint64_t)n * B;
     const bool bfs = (algo == EEG_BFS);
@@ -1563,20 +1571,30 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
     }
     // answers: query-major, original vertex order
     const uint64_t outn = (uint64_t)nsrc * n;
-    if (out) {
+    if (bo.out) {
         uint32_t* dst = nullptr;
         DevMem tmp;
-        if (out_device) {
-            dst = (uint32_t*)out;
+        if (bo.device) {
+            dst = (uint32_t*)bo.out;
         } else {
             EEG_TRY(tmp.alloc(outn * 4 + 4));
             dst = tmp.as<uint32_t>();
         }
-        k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B,
+        k_planes_out<<<grid_n(outn, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B, 0u,
                                                         nsrc, lm, dst);
         EEG_HIP(hipGetLastError());
-        if (!out_device) EEG_HIP(hipMemcpyAsync(out, dst, outn * 4, hipMemcpyDeviceToHost, s));
+        if (!bo.device) EEG_HIP(hipMemcpyAsync(bo.out, dst, outn * 4, hipMemcpyDeviceToHost, s));
         EEG_HIP(hipStreamSynchronize(s));
+    } else if (bo.rows) {  // one row at a time through one device row
+        EEG_TRY(bw.rowbuf.alloc((size_t)n * 4 + 4));
+        for (uint32_t q = 0; q < nsrc; ++q) {
+            if (!bo.rows[q]) continue;
+            k_planes_out<<<grid_n(n, 65536), WG, 0, s>>>(bw.planes.as<uint32_t>(), g->perm.as<uint32_t>(), n, B, q, 1u,
+                                                         lm, bw.rowbuf.as<uint32_t>());
+            EEG_HIP(hipGetLastError());
+            EEG_HIP(hipMemcpyAsync(bo.rows[q], bw.rowbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            EEG_HIP(hipStreamSynchronize(s));
+        }
     }
     EEG_HIP(hipStreamSynchronize(s));
     float ms = 0;
@@ -1599,28 +1617,29 @@ eeg_status batch_pass(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint
 namespace {
 
 // A pass on lm labels; a 16-bit pass that overflows is answered again on 32-bit labels.
-eeg_status pass_or_wider(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
-                         void* out, bool out_device, eeg_stats* st, int lm, uint64_t ovq[MAXW]) {
+eeg_status pass_or_wider(eeg_graph* g, BatchWork& bw, hipStream_t s, uint32_t algo, const uint32_t* sources,
+                         uint32_t nsrc, const eeg_opts& o, const BatchOut& bo, eeg_stats* st, int lm,
+                         uint64_t ovq[MAXW]) {
     bool overflow = false;
-    EEG_TRY(batch_pass(g, algo, sources, nsrc, o, out, out_device, st, lm, &overflow, ovq));
+    EEG_TRY(batch_pass(g, bw, s, algo, sources, nsrc, o, bo, st, lm, &overflow, ovq));
     if (!overflow) return EEG_OK;
     overflow = false;
-    return batch_pass(g, algo, sources, nsrc, o, out, out_device, st, LM_U32, &overflow, ovq);
+    return batch_pass(g, bw, s, algo, sources, nsrc, o, bo, st, LM_U32, &overflow, ovq);
 }
 
-}  // namespace
-
-eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
-                     void* out, bool out_device, eeg_stats* st) {
+// Multi-source queries on the state bw, stream s and (for single-query reruns) traversal state
+// wk (null: the graph's own state, whose stream must then be s).
+eeg_status run_batch_core(eeg_graph* g, BatchWork& bw, Work* wk, hipStream_t s, uint32_t algo,
+                          const uint32_t* sources, uint32_t nsrc, const eeg_opts& o, const BatchOut& bo,
+                          eeg_stats* st) {
     const auto t0 = std::chrono::steady_clock::now();
-    BatchWork& bw = batch_work(g);
     int lm = LM_U32;
     if ((algo == EEG_SSSP || algo == EEG_SSWP) && g->weighted && nsrc > PLANE_DENSE_FROM && nsrc <= 64 * MAXW) {
         if (o.label_bits != 0 && o.label_bits != 32)
             return fail(EEG_ERR_INVALID_ARG, "label_bits must be 0 or 32 (got %u)", o.label_bits);
         if (o.label_bits == 0) {
             uint32_t wmax = 0;
-            EEG_TRY(graph_wmax(g, bw, &wmax));
+            EEG_TRY(graph_wmax(g, bw, s, &wmax));
             if (algo == EEG_SSSP) {
                 // 8-bit distances while the weights leave room for a few hops under 255; a
                 // graph whose 8-bit passes flagged many queries goes straight to 16 bits
@@ -1631,10 +1650,10 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         }
     }
     uint64_t ovq[MAXW];
-    EEG_TRY(pass_or_wider(g, algo, sources, nsrc, o, out, out_device, st, lm, ovq));
+    EEG_TRY(pass_or_wider(g, bw, s, algo, sources, nsrc, o, bo, st, lm, ovq));
     if (lm == LM_D8) {
-        // queries whose distances outgrew 8 bits: answered again on 16-bit planes (32-bit when
-        // they are few or outgrow those too), their rows copied over the 8-bit answers
+        // queries whose distances outgrew 8 bits: answered again, their rows written over the
+        // 8-bit answers
         std::vector<uint32_t> qs, srcs;
         for (uint32_t q = 0; q < nsrc; ++q)
             if ((ovq[q / 64] >> (q % 64)) & 1ull) {
@@ -1646,16 +1665,6 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
         if (!qs.empty()) {
             const uint32_t nr = (uint32_t)qs.size(), n = g->n;
             const uint64_t row = (uint64_t)n * 4;
-            DevMem dtmp;
-            std::vector<uint32_t> htmp;
-            void* tmp = nullptr;
-            if (out && out_device) {
-                EEG_TRY(dtmp.alloc(nr * row + 4));
-                tmp = dtmp.p;
-            } else if (out) {
-                htmp.resize((size_t)nr * n);
-                tmp = htmp.data();
-            }
             eeg_stats s2;
             memset(&s2, 0, sizeof(s2));
             if (nr <= 8) {
@@ -1666,35 +1675,47 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
                 o1.variant = EEG_SYNC_PUSH_DD;
                 o1.frontier = EEG_FRONTIER_SORTED;
                 o1.K = o.K;
-                DevMem d1;
-                if (out && !out_device) EEG_TRY(d1.alloc(row + 4));
+                Work& w = wk ? *wk : g->work;
+                const bool host = bo.rows || (bo.out && !bo.device);
+                if (host) EEG_TRY(bw.rowbuf.alloc(row + 4));
                 for (uint32_t i = 0; i < nr; ++i) {
                     eeg_stats s1;
                     memset(&s1, 0, sizeof(s1));
-                    EEG_TRY(run_traversal(g, algo, g->h_perm[srcs[i]], o1, &s1));
+                    EEG_TRY(run_traversal(g, algo, g->h_perm[srcs[i]], o1, &s1, nullptr, 0, wk, wk ? s : nullptr));
                     s2.ms_traversal += s1.ms_traversal;
                     s2.launches += s1.launches;
-                    if (!out) continue;
-                    void* dst = out_device ? (void*)((uint8_t*)out + (uint64_t)qs[i] * row) : d1.p;
-                    EEG_TRY(unpermute(g, g->work.lab.p, dst, g->stream));
-                    if (!out_device)
-                        EEG_HIP(hipMemcpyAsync((uint8_t*)out + (uint64_t)qs[i] * row, d1.p, row, hipMemcpyDeviceToHost,
-                                               g->stream));
-                    EEG_HIP(hipStreamSynchronize(g->stream));
+                    void* hdst = bo.rows ? (void*)bo.rows[qs[i]]
+                                         : bo.out ? (void*)((uint8_t*)bo.out + (uint64_t)qs[i] * row) : nullptr;
+                    if (!hdst) continue;
+                    EEG_TRY(unpermute(g, w.lab.p, host ? bw.rowbuf.p : hdst, s));
+                    if (host) EEG_HIP(hipMemcpyAsync(hdst, bw.rowbuf.p, row, hipMemcpyDeviceToHost, s));
+                    EEG_HIP(hipStreamSynchronize(s));
                 }
-                out = nullptr;  // rows written
             } else {
+                // more: one pass on 16-bit planes (32-bit for 16 or fewer), answers straight to
+                // the flagged queries' rows
                 const int lm2 = nr > PLANE_DENSE_FROM ? LM_D16 : LM_U32;
-                EEG_TRY(pass_or_wider(g, algo, srcs.data(), nr, o, tmp, out_device, &s2, lm2, nullptr));
-            }
-            if (out) {
-                for (uint32_t i = 0; i < nr; ++i) {
-                    uint8_t* dst = (uint8_t*)out + (uint64_t)qs[i] * row;
-                    const uint8_t* src = (const uint8_t*)tmp + (uint64_t)i * row;
-                    if (out_device) EEG_HIP(hipMemcpyAsync(dst, src, row, hipMemcpyDeviceToDevice, g->stream));
-                    else memcpy(dst, src, row);
+                std::vector<uint32_t*> sub(nr, nullptr);
+                DevMem dtmp;
+                BatchOut b2;
+                if (bo.rows) {
+                    for (uint32_t i = 0; i < nr; ++i) sub[i] = bo.rows[qs[i]];
+                    b2.rows = sub.data();
+                } else if (bo.out && bo.device) {
+                    EEG_TRY(dtmp.alloc(nr * row + 4));
+                    b2.out = dtmp.p;
+                    b2.device = true;
+                } else if (bo.out) {
+                    for (uint32_t i = 0; i < nr; ++i) sub[i] = (uint32_t*)((uint8_t*)bo.out + (uint64_t)qs[i] * row);
+                    b2.rows = sub.data();
                 }
-                if (out_device) EEG_HIP(hipStreamSynchronize(g->stream));
+                EEG_TRY(pass_or_wider(g, bw, s, algo, srcs.data(), nr, o, b2, &s2, lm2, nullptr));
+                if (b2.device) {
+                    for (uint32_t i = 0; i < nr; ++i)
+                        EEG_HIP(hipMemcpyAsync((uint8_t*)bo.out + (uint64_t)qs[i] * row, (uint8_t*)dtmp.p + (uint64_t)i * row,
+                                               row, hipMemcpyDeviceToDevice, s));
+                    EEG_HIP(hipStreamSynchronize(s));
+                }
             }
             if (st) {
                 st->ms_traversal += s2.ms_traversal;
@@ -1704,6 +1725,26 @@ eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint3
     }
     if (st) st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return EEG_OK;
+}
+
+}  // namespace
+
+eeg_status run_batch(eeg_graph* g, uint32_t algo, const uint32_t* sources, uint32_t nsrc, const eeg_opts& o,
+                     void* out, bool out_device, eeg_stats* st) {
+    BatchOut bo;
+    bo.out = out;
+    bo.device = out_device;
+    return run_batch_core(g, batch_work(g), nullptr, g->stream, algo, sources, nsrc, o, bo, st);
+}
+
+std::shared_ptr<BatchWork> make_batch_work() { return std::make_shared<BatchWork>(); }
+
+eeg_status run_batch_rows(eeg_graph* g, BatchWork& bw, Work& wk, hipStream_t s, uint32_t algo,
+                          const uint32_t* sources, uint32_t nsrc, const eeg_opts& o, uint32_t* const* rows,
+                          eeg_stats* st) {
+    BatchOut bo;
+    bo.rows = rows;
+    return run_batch_core(g, bw, &wk, s, algo, sources, nsrc, o, bo, st);
 }
 
 eeg_status launch_ms_level(MsArgs a, uint32_t K, unsigned grid, hipStream_t s) {

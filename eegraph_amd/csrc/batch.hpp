@@ -2,11 +2,26 @@
 // serving engines (serve.hip): kernel argument blocks and host-side launch wrappers.
 #pragma once
 
+#include <memory>
+
 #include "engine.hpp"
 
 namespace eeg {
 
 constexpr uint32_t BATCH_MAXW = 4;  // up to 256 queries per traversal (4 x 64-bit words)
+
+// State of multi-source passes (label planes, masks, queues, the replayed graph of a chunk);
+// one per graph handle for eeg_run_batch, one per serving lane that batches its queries.
+struct BatchWork;
+std::shared_ptr<BatchWork> make_batch_work();
+// BFS/SSSP/SSWP from up to 256 sources in one pass (narrow labels as eeg_run_batch chooses
+// them) on the state bw, stream s and, for the single-query reruns of 8-bit distances, the
+// traversal state wk (also on s).  Row q of the answers (n words, original vertex order) goes
+// to the host memory rows[q]; null rows are not copied out.  The graph's tables for K must
+// exist.
+eeg_status run_batch_rows(eeg_graph* g, BatchWork& bw, Work& wk, hipStream_t s, uint32_t algo,
+                          const uint32_t* sources, uint32_t nsrc, const eeg_opts& o, uint32_t* const* rows,
+                          eeg_stats* st);
 
 // Multi-source BFS with 64-bit visit masks: a push launch (odd launch index) ORs new bits into
 // `next`, a finalize launch (even) turns next & ~seen into the next frontier and levels.

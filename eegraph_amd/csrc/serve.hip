@@ -171,6 +171,7 @@ struct Lane {
     Work w;
     hipStream_t st = nullptr;
     DevMem out;
+    std::shared_ptr<BatchWork> bw;  // multi-source passes of this lane (made on first use)
     std::thread th;
     ~Lane() {
         if (st) (void)hipStreamDestroy(st);
@@ -197,6 +198,7 @@ struct eeg_server {
     std::deque<eeg::TicketP> lane_q;
     std::condition_variable lane_cv;
     uint32_t lanes_busy = 0;  // lanes running a query (guarded by mu)
+    bool lane_batch_off = false;  // a lane's multi-source pass found no memory (guarded by mu)
     std::vector<std::unique_ptr<eeg::Lane>> lanes;
     std::mutex prep_mu;  // builds of the derived tables a lane needs
     // answer extraction (labels of a finished column to the caller)
@@ -771,11 +773,63 @@ eeg_status prepare_for(eeg_server* srv, uint32_t algo) {
     return ensure_tigr(g, g->out, g->tout, srv->K);
 }
 
+// Up to 256 waiting SSSP/SSWP queries answered in one multi-source pass on the lane's own
+// state; false when the pass found no device memory (the queries are handed back to the queue).
+bool lane_batch(eeg_server* srv, Lane* ln, std::vector<TicketP>& batch) {
+    eeg_graph* g = srv->g;
+    const uint32_t a = batch[0]->algo, k = (uint32_t)batch.size(), n = g->n;
+    eeg_opts o;
+    memset(&o, 0, sizeof(o));
+    o.variant = EEG_SYNC_PUSH_DD;
+    o.K = srv->K;
+    std::vector<uint32_t> srcs(k);
+    for (uint32_t i = 0; i < k; ++i) srcs[i] = batch[i]->source;
+    std::vector<std::vector<uint32_t>> outs(k);
+    std::vector<uint32_t*> rows(k, nullptr);
+    if (!srv->discard)
+        for (uint32_t i = 0; i < k; ++i) {
+            outs[i].resize(n);
+            rows[i] = outs[i].data();
+        }
+    eeg_status rc = prepare_for(srv, a);
+    if (rc == EEG_OK && !ln->bw) ln->bw = make_batch_work();
+    if (rc == EEG_OK) rc = run_batch_rows(g, *ln->bw, ln->w, ln->st, a, srcs.data(), k, o, rows.data(), nullptr);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    --srv->lanes_busy;
+    if (rc == EEG_ERR_OOM) {  // answer them one at a time from now on
+        ln->bw.reset();
+        srv->lane_batch_off = true;
+        for (auto it = batch.rbegin(); it != batch.rend(); ++it) {
+            (*it)->state = T_PENDING;
+            srv->lane_q.push_front(*it);
+        }
+        srv->lane_cv.notify_all();
+        return false;
+    }
+    const double tnow = srv->now_ms();
+    const std::string err = rc != EEG_OK ? eeg_last_error() : "";
+    for (uint32_t i = 0; i < k; ++i) {
+        TicketP& tk = batch[i];
+        tk->status = rc;
+        if (rc != EEG_OK) tk->err = err;
+        tk->state = T_DONE;
+        tk->t_done = tnow;
+        tk->host.swap(outs[i]);
+    }
+    srv->cv_done.notify_all();
+    return true;
+}
+
 void lane_loop(eeg_server* srv, Lane* ln) {
     eeg_graph* g = srv->g;
     (void)hipSetDevice(g->device);
+    // a multi-source pass costs several single queries, even with few sources (RMAT-26: a
+    // pass of 17 sources held SSSP p50 at 1.8 s at 200 offered q/s against 0.8 s one at a
+    // time), so a lane batches only a long queue
+    constexpr size_t kLaneBatchMin = 64, kLaneBatchMax = 64 * BATCH_MAXW;
     while (true) {
         TicketP tk;
+        std::vector<TicketP> batch;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             srv->lane_cv.wait(lk, [&] { return srv->stop || !srv->lane_q.empty(); });
@@ -792,6 +846,39 @@ void lane_loop(eeg_server* srv, Lane* ln) {
             tk->t_start = srv->now_ms();
             tk->batch = 1;
             ++srv->lanes_busy;
+            if ((tk->algo == EEG_SSSP || tk->algo == EEG_SSWP) && !srv->lane_batch_off &&
+                (srv->cfg.flags & EEG_SERVER_NO_LANE_BATCH) == 0) {
+                size_t same = 1;
+                for (const TicketP& t : srv->lane_q)
+                    if (t->algo == tk->algo && !t->cancel) ++same;
+                if (same >= kLaneBatchMin) {
+                    batch.push_back(tk);
+                    bool cancelled = false;
+                    for (auto it = srv->lane_q.begin(); it != srv->lane_q.end() && batch.size() < kLaneBatchMax;) {
+                        if ((*it)->algo != tk->algo) {
+                            ++it;
+                            continue;
+                        }
+                        TicketP t = *it;
+                        it = srv->lane_q.erase(it);
+                        if (t->cancel) {
+                            t->state = T_CANCELLED;
+                            t->status = EEG_ERR_CANCELLED;
+                            cancelled = true;
+                            continue;
+                        }
+                        t->state = T_RUNNING;
+                        t->t_start = tk->t_start;
+                        batch.push_back(t);
+                    }
+                    for (TicketP& t : batch) t->batch = (uint32_t)batch.size();
+                    if (cancelled) srv->cv_done.notify_all();
+                }
+            }
+        }
+        if (!batch.empty()) {
+            lane_batch(srv, ln, batch);
+            continue;
         }
         eeg_opts o;
         memset(&o, 0, sizeof(o));
@@ -847,8 +934,8 @@ extern "C" eeg_status eeg_server_create(eeg_graph* g, const eeg_server_cfg* cfg,
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32");
     if (c.damping == 0.0f) c.damping = 0.85f;
     if (c.ppr_epsilon == 0.0) c.ppr_epsilon = 1e-7;
-    const uint32_t known =
-        EEG_SERVER_DISCARD_LABELS | EEG_SERVER_PLANE_ENGINES | EEG_SERVER_NO_BFS_ENGINE | EEG_SERVER_NO_PPR_ENGINE;
+    const uint32_t known = EEG_SERVER_DISCARD_LABELS | EEG_SERVER_PLANE_ENGINES | EEG_SERVER_NO_BFS_ENGINE |
+                           EEG_SERVER_NO_PPR_ENGINE | EEG_SERVER_NO_LANE_BATCH;
     if (c.flags & ~known) return fail(EEG_ERR_INVALID_ARG, "unknown flags 0x%x", c.flags);
     if (c.lanes > 64) return fail(EEG_ERR_INVALID_ARG, "lanes must be <= 64");
     DeviceGuard dg;

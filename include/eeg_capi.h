@@ -291,7 +291,8 @@ eeg_status eeg_autotune(eeg_graph* g, uint32_t algo, const uint32_t* sample_sour
  * (admission by label memory) and join the next iteration of the pass already running
  * (continuous batching); the column is given back at convergence, the answer kept in a result
  * buffer until it is read.  Other queries run on `lanes` concurrent single-query lanes whose
- * launches overlap on the device.  Every answer equals eeg_run's for the same query (PPR:
+ * launches overlap on the device; a lane batches waiting SSSP/SSWP queries (see
+ * EEG_SERVER_NO_LANE_BATCH).  Every answer equals eeg_run's for the same query (PPR:
  * within the stated tolerance, the same iteration summed in another fixed order).  The graph
  * must outlive the server; eeg_run calls on the same graph from other threads are allowed. */
 typedef struct eeg_server_cfg {
@@ -325,6 +326,10 @@ typedef struct eeg_server_cfg {
 /* PPR goes through lanes instead of the PPR engine (rank planes of 8, 16 or 32 restart
  * vertices, one gather pass per iteration for every query in flight). */
 #define EEG_SERVER_NO_PPR_ENGINE 8u
+/* Lanes answer SSSP and SSWP one query at a time.  By default a lane that takes an SSSP or
+ * SSWP query while 63 or more others of the same algorithm wait answers up to 256 of them in
+ * one multi-source pass (8-bit label planes when the weights allow, as eeg_run_batch). */
+#define EEG_SERVER_NO_LANE_BATCH 16u
 typedef struct eeg_query_result {
     uint32_t algo;
     uint32_t source;

@@ -199,3 +199,32 @@ def test_ppr_engine_columns_join_and_leave(rmat12, columns):
         _check(g, "ppr", x, lab, n, row, col, singles)
         shared += r["batch_size"] > 1
     assert shared > 0
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_lanes_batch_waiting_sssp_sswp(rmat12, lanes):
+    """A lane that takes an SSSP or SSWP query while 63 or more of the same algorithm wait
+    answers up to 256 of them in one multi-source pass: 150 SSSP and 100 SSWP queries (and some
+    BFS on lanes between them) submitted at once, every answer equal to its single run, and the
+    batch size reported.  The same stream with lane batching off answers one at a time."""
+    n, s, d, w, g, row, col = rmat12
+    rng = np.random.default_rng(31 + lanes)
+    mix = ["sssp"] * 150 + ["sswp"] * 100 + ["bfs"] * 10
+    rng.shuffle(mix)
+    srcs = rng.integers(0, n, size=len(mix))
+    srcs[::9] = g.info()["argmax_out_degree"]
+    singles = {}
+    for batch in (True, False):
+        with eg.Server(g, lanes=lanes, bfs_engine=False, ppr_engine=False, lane_batch=batch) as srv:
+            tickets = [srv.submit(a, int(x)) for a, x in zip(mix, srcs)]
+            sizes = []
+            for a, x, t in zip(mix, srcs, tickets):
+                lab, r = srv.wait(t, algo=a)
+                assert r["status"] == L.OK
+                _check(g, a, int(x), lab, n, row, col, singles)
+                if a != "bfs":
+                    sizes.append(r["batch_size"])
+        if batch:
+            assert max(sizes) >= 64 and max(sizes) <= 256, max(sizes)
+        else:
+            assert max(sizes) == 1
