@@ -823,10 +823,10 @@ bool lane_batch(eeg_server* srv, Lane* ln, std::vector<TicketP>& batch) {
 void lane_loop(eeg_server* srv, Lane* ln) {
     eeg_graph* g = srv->g;
     (void)hipSetDevice(g->device);
-    // a multi-source pass costs several single queries, even with few sources (RMAT-26: a
-    // pass of 17 sources held SSSP p50 at 1.8 s at 200 offered q/s against 0.8 s one at a
-    // time), so a lane batches only a long queue
-    constexpr size_t kLaneBatchMin = 64, kLaneBatchMax = 64 * BATCH_MAXW;
+    // a multi-source pass costs several single queries, even with few sources, so a lane
+    // batches only a queue of 32 or more (RMAT-26 mix: 17 raised the SSSP p50 at 200 offered
+    // q/s, 32 served 1.90-2.16x at 200-1000 q/s against 1.80-2.02x for 64)
+    constexpr size_t kLaneBatchMin = 32, kLaneBatchMax = 64 * BATCH_MAXW;
     while (true) {
         TicketP tk;
         std::vector<TicketP> batch;

@@ -327,7 +327,7 @@ typedef struct eeg_server_cfg {
  * vertices, one gather pass per iteration for every query in flight). */
 #define EEG_SERVER_NO_PPR_ENGINE 8u
 /* Lanes answer SSSP and SSWP one query at a time.  By default a lane that takes an SSSP or
- * SSWP query while 63 or more others of the same algorithm wait answers up to 256 of them in
+ * SSWP query while 31 or more others of the same algorithm wait answers up to 256 of them in
  * one multi-source pass (8-bit label planes when the weights allow, as eeg_run_batch). */
 #define EEG_SERVER_NO_LANE_BATCH 16u
 typedef struct eeg_query_result {
