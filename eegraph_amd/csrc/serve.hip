@@ -38,6 +38,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -786,17 +787,25 @@ bool lane_batch(eeg_server* srv, Lane* ln, std::vector<TicketP>& batch) {
     for (uint32_t i = 0; i < k; ++i) srcs[i] = batch[i]->source;
     std::vector<std::vector<uint32_t>> outs(k);
     std::vector<uint32_t*> rows(k, nullptr);
-    if (!srv->discard)
-        for (uint32_t i = 0; i < k; ++i) {
-            outs[i].resize(n);
-            rows[i] = outs[i].data();
+    eeg_status rc = EEG_OK;
+    if (!srv->discard) {
+        try {  // k answers of n words each wait in host memory for their clients
+            for (uint32_t i = 0; i < k; ++i) {
+                outs[i].resize(n);
+                rows[i] = outs[i].data();
+            }
+        } catch (const std::bad_alloc&) {
+            outs.clear();
+            outs.resize(k);
+            rc = fail(EEG_ERR_OOM, "no host memory for %u answers", k);
         }
-    eeg_status rc = prepare_for(srv, a);
+    }
+    if (rc == EEG_OK) rc = prepare_for(srv, a);
     if (rc == EEG_OK && !ln->bw) ln->bw = make_batch_work();
     if (rc == EEG_OK) rc = run_batch_rows(g, *ln->bw, ln->w, ln->st, a, srcs.data(), k, o, rows.data(), nullptr);
     std::lock_guard<std::mutex> lk(srv->mu);
     --srv->lanes_busy;
-    if (rc == EEG_ERR_OOM) {  // answer them one at a time from now on
+    if (rc == EEG_ERR_OOM) {  // device or host memory ran short: one at a time from now on
         ln->bw.reset();
         srv->lane_batch_off = true;
         for (auto it = batch.rbegin(); it != batch.rend(); ++it) {
