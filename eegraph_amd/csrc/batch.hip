@@ -345,9 +345,8 @@ __global__ __launch_bounds__(WG) void k_ms_finalize(MsArgs args) {
 //           <= 254 is found as above, and a vertex at a finite distance >= 255 would have a
 //           first such vertex y on its shortest path, whose predecessor x (distance <= 254,
 //           final and pushed) offers y a candidate >= 255 while y is unreached (a label is
-//           never below the true distance).  A flagged query still runs to the end on 8-bit
-//           labels, which leaves its distances up to 254 exact; the caller continues it from
-//           there with 32-bit labels (a few queries) or answers it again on 16-bit planes.
+//           never below the true distance).  The caller answers the flagged queries again
+//           with wider labels.
 // Improvements of a lane word are compare-and-swap loops: labels only improve, so every
 // failed swap means another improvement landed and the loop ends.
 enum LabelMode : int { LM_U32 = 0, LM_W8 = 1, LM_D16 = 2, LM_D8 = 3 };
@@ -423,6 +422,7 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
     __shared__ uint32_t sQ[TILE_EDGES];
     __shared__ uint32_t sMisc[8];
     __shared__ uint64_t sScan[WG / WAVE + 1];
+    __shared__ uint64_t sOv[MAXW];  // 8-bit distances: flagged queries, which stop pushing
     Ctl* ctl = a.ctl;
     if (a.act && blockIdx.x == 0 && threadIdx.x < MAXW) a.act[(a.it % 4) * MAXW + threadIdx.x] = 0;
     if (ctl->done) return;
@@ -459,6 +459,11 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
             sMisc[0] = 1;
             sMisc[1] = 1;
             sMisc[2] = 0;
+        }
+        if constexpr (LM == LM_D8) {
+            // a flagged query is answered again wider, so its further pushes are wasted work
+            // (its column only feeds itself); a stale copy only costs some of that work
+            if (threadIdx.x < MAXW) sOv[threadIdx.x] = __builtin_nontemporal_load(&a.ovq[threadIdx.x]);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < emax; k += WG) {
@@ -505,6 +510,7 @@ __global__ __launch_bounds__(WG, K <= 8 ? (LM == LM_U32 ? 7 : 8) : 1) void k_pla
                     const uint32_t x = x0 + (uint32_t)lane;
                     uint32_t nib = 0;
                     if (x < nl) nib = (uint32_t)(a.chg[(uint64_t)v * a.W + x / GL] >> (QPL * (x % GL))) & GM;
+                    if constexpr (LM == LM_D8) nib &= ~(uint32_t)(sOv[x / GL] >> (QPL * (x % GL)));
                     if (!__ballot(nib != 0)) continue;  // the same answer in every lane
                     const uint32_t lv = nib ? plane[(uint64_t)v * nl + x] : 0u;
                     uint32_t ovf = 0;  // D16: candidates past 16 bits; D8: flagged queries
@@ -977,18 +983,6 @@ __global__ void k_max_u32(const uint32_t* __restrict__ a, uint64_t n, uint32_t* 
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, WAVE));
     if (lane_id() == 0 && m) atomicMax(out, m);
-}
-
-// Query q of an 8-bit distance plane as 32-bit labels in internal order (INF for 255), and the
-// vertices that may have offered a dropped candidate (distance >= 255 - wmax, finite) appended
-// to seeds (*count is zeroed by the caller; at most n entries).
-__global__ void k_warm_column(const uint8_t* __restrict__ plane, uint32_t n, uint32_t B, uint32_t q, uint32_t wmax,
-                              uint32_t* __restrict__ lab, uint32_t* __restrict__ seeds, uint32_t* __restrict__ count) {
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x = plane[v * B + q];
-        lab[v] = x == 0xffu ? INF : x;
-        if (x != 0xffu && x + wmax >= 0xffu) seeds[atomicAdd(count, 1u)] = (uint32_t)v;
-    }
 }
 
 // Fills with 16-byte stores where the base allows (the planes are many GB).
@@ -1640,11 +1634,11 @@ eeg_status run_batch_core(eeg_graph* g, BatchWork& bw, Work* wk, hipStream_t s, 
                           eeg_stats* st) {
     const auto t0 = std::chrono::steady_clock::now();
     int lm = LM_U32;
-    uint32_t wmax = 0;
     if ((algo == EEG_SSSP || algo == EEG_SSWP) && g->weighted && nsrc > PLANE_DENSE_FROM && nsrc <= 64 * MAXW) {
         if (o.label_bits != 0 && o.label_bits != 32)
             return fail(EEG_ERR_INVALID_ARG, "label_bits must be 0 or 32 (got %u)", o.label_bits);
         if (o.label_bits == 0) {
+            uint32_t wmax = 0;
             EEG_TRY(graph_wmax(g, bw, s, &wmax));
             if (algo == EEG_SSSP) {
                 // 8-bit distances while the weights leave room for a few hops under 255; a
@@ -1674,11 +1668,8 @@ eeg_status run_batch_core(eeg_graph* g, BatchWork& bw, Work* wk, hipStream_t s, 
             eeg_stats s2;
             memset(&s2, 0, sizeof(s2));
             if (nr <= 8) {
-                // a few: single-query traversals (sync push on sorted queues) continued from
-                // the 8-bit answer.  Its labels up to 254 are exact; a vertex past 254 has a
-                // first such vertex on its shortest path, whose predecessor is at 255 - wmax
-                // or more, so those vertices are the first frontier (RMAT-24: ~0.3 ms instead
-                // of a 4.6 ms run from the source)
+                // a few: single-query traversals (sync push on sorted queues), each about a
+                // third of a one-source pass
                 eeg_opts o1;
                 memset(&o1, 0, sizeof(o1));
                 o1.variant = EEG_SYNC_PUSH_DD;
@@ -1687,27 +1678,10 @@ eeg_status run_batch_core(eeg_graph* g, BatchWork& bw, Work* wk, hipStream_t s, 
                 Work& w = wk ? *wk : g->work;
                 const bool host = bo.rows || (bo.out && !bo.device);
                 if (host) EEG_TRY(bw.rowbuf.alloc(row + 4));
-                DevMem dinit, dseeds, dcnt;
-                EEG_TRY(dinit.alloc(row + 4));
-                EEG_TRY(dseeds.alloc(row + 4));
-                EEG_TRY(dcnt.alloc(8));
-                const uint32_t B8 = ((nsrc + 63) / 64) * 64;
                 for (uint32_t i = 0; i < nr; ++i) {
                     eeg_stats s1;
                     memset(&s1, 0, sizeof(s1));
-                    k_fill_u32b<<<1, 64, 0, s>>>(dcnt.as<uint32_t>(), 1, 0u);
-                    k_warm_column<<<grid_n(n, 65536), WG, 0, s>>>(bw.planes.as<uint8_t>(), n, B8, qs[i], wmax,
-                                                                  dinit.as<uint32_t>(), dseeds.as<uint32_t>(),
-                                                                  dcnt.as<uint32_t>());
-                    EEG_HIP(hipGetLastError());
-                    uint32_t nseed = 0;
-                    EEG_HIP(hipMemcpyAsync(&nseed, dcnt.p, 4, hipMemcpyDeviceToHost, s));
-                    EEG_HIP(hipStreamSynchronize(s));
-                    if (nseed > 0)
-                        EEG_TRY(run_traversal(g, algo, 0, o1, &s1, dseeds.as<uint32_t>(), nseed, wk, wk ? s : nullptr,
-                                              dinit.as<uint32_t>()));
-                    else  // not reached (a flag implies such a vertex): the whole query again
-                        EEG_TRY(run_traversal(g, algo, g->h_perm[srcs[i]], o1, &s1, nullptr, 0, wk, wk ? s : nullptr));
+                    EEG_TRY(run_traversal(g, algo, g->h_perm[srcs[i]], o1, &s1, nullptr, 0, wk, wk ? s : nullptr));
                     s2.ms_traversal += s1.ms_traversal;
                     s2.launches += s1.launches;
                     void* hdst = bo.rows ? (void*)bo.rows[qs[i]]

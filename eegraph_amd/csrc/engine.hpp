@@ -205,11 +205,9 @@ eeg_status generate_rmat_device(uint32_t scale, uint32_t ef, double a, double b,
 // `source` is an internal id; d_seeds (device, nseeds distinct internal ids) replaces it when
 // given.  The query runs with the graph's own state and stream, or with `wk` on `sk` (several
 // queries at once on one graph; the tables they read must have been built beforehand).
-// d_init (device, n labels in internal order) starts the labels from a partial answer instead
-// of the source's; the seeds are then only the first frontier (BFS/SSSP/SSWP).
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                          eeg_stats* st, const uint32_t* d_seeds = nullptr, uint32_t nseeds = 0,
-                         Work* wk = nullptr, hipStream_t sk = nullptr, const uint32_t* d_init = nullptr);
+                         Work* wk = nullptr, hipStream_t sk = nullptr);
 eeg_status run_pagerank(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
                         eeg_stats* st, Work* wk = nullptr, hipStream_t sk = nullptr);
 

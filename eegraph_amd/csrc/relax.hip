@@ -713,8 +713,7 @@ eeg_status chunk_graph(Work& w, KernelFn kfn, unsigned grid, unsigned block, con
 }  // namespace
 
 eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg_opts& o,
-                         eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds, Work* wk, hipStream_t sk,
-                         const uint32_t* d_init) {
+                         eeg_stats* st, const uint32_t* d_seeds, uint32_t nseeds, Work* wk, hipStream_t sk) {
     const auto t_host0 = std::chrono::steady_clock::now();
     const uint32_t K = o.K ? o.K : 8;
     if (k_index(K) < 0) return fail(EEG_ERR_INVALID_ARG, "K must be 4, 8, 16 or 32 (got %u)", K);
@@ -793,9 +792,7 @@ eeg_status run_traversal(eeg_graph* g, uint32_t algo, uint32_t source, const eeg
         k_init_labels<<<grid_for(n, 8192), WG, 0, s>>>(w.lab.as<uint32_t>(), n, fill, at_src,
                                                        d_seeds ? 0xffffffffu : source,
                                                        alg == A_CC ? g->inv.as<uint32_t>() : nullptr);
-        if (d_init && alg != A_CC)
-            EEG_HIP(hipMemcpyAsync(w.lab.p, d_init, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-        else if (d_seeds && alg != A_CC && nseeds)
+        if (d_seeds && alg != A_CC && nseeds)
             k_seed_labels<<<grid_for(nseeds, 1024), WG, 0, s>>>(w.lab.as<uint32_t>(), d_seeds, nseeds, at_src);
     }
     EEG_HIP(hipGetLastError());
