@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--serve-queries", type=int, default=200, help="queries per serving point")
     p.add_argument("--serve-ppr-columns", type=int, default=0,
                    help="PPR engine width of the serving section (8, 16 or 32; 0 = the server's default)")
+    p.add_argument("--serve-lanes", type=int, default=0,
+                   help="single-query lanes of the serving section (0 = the server's default, 2)")
     p.add_argument("--serve-queries-top", type=int, default=1500,
                    help="queries at the highest offered load (the saturated throughput point; long enough "
                         "that the last queries' latency does not set the rate: 600 measured 2.3-2.45x, "
@@ -529,7 +531,7 @@ def serving_section(eg, a):
     one_j = round(p1[0] * (t2 - t1) / nmax, 4) if p1 else None
     log(f"one at a time: {nmax} queries in {t2 - t1:.2f} s ({nmax / (t2 - t1):.1f} q/s)")
 
-    srv = eg.Server(g, discard_labels=True, ppr_columns=a.serve_ppr_columns)
+    srv = eg.Server(g, discard_labels=True, ppr_columns=a.serve_ppr_columns, lanes=a.serve_lanes)
     for alg in ("bfs", "sssp", "sswp", "ppr"):  # warm every path once
         srv.wait(srv.submit(alg, 0))
     points = []
