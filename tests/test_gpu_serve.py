@@ -228,3 +228,30 @@ def test_lanes_batch_waiting_sssp_sswp(rmat12, lanes):
             assert max(sizes) >= 32 and max(sizes) <= 256, max(sizes)
         else:
             assert max(sizes) == 1
+
+
+def test_lane_batches_with_cancelled_queries(rmat12):
+    """Queries cancelled while queued never join a lane batch (one a lane already took is
+    answered); the rest of a long SSSP queue is answered in batches, each answer equal to its
+    single run."""
+    n, s, d, w, g, row, col = rmat12
+    srcs = np.random.default_rng(41).integers(0, n, size=120)
+    singles = {}
+    with eg.Server(g, lanes=1, bfs_engine=False, ppr_engine=False) as srv:
+        tk = [srv.submit("sssp", int(x)) for x in srcs]
+        gone = set(range(3, 120, 5))
+        for i in gone:
+            srv.cancel(tk[i])
+        sizes = []
+        for i, (t, x) in enumerate(zip(tk, srcs)):
+            if i in gone:
+                try:
+                    lab, r = srv.wait(t, algo="sssp")
+                except eg.EEGError:
+                    continue  # cancelled in the queue
+            else:
+                lab, r = srv.wait(t, algo="sssp")
+            assert r["status"] == L.OK
+            _check(g, "sssp", int(x), lab, n, row, col, singles)
+            sizes.append(r["batch_size"])
+        assert max(sizes) >= 32
