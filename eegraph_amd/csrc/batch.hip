@@ -357,45 +357,9 @@ struct Packed;
 // Widest-path labels, four per word (bytes8.hpp).
 template <>
 struct Packed<LM_W8> : Bytes8Widest {};
-// Distances of BITS bits per label, all ones = INF.  cand() sets bit i of `ov` for a query
-// whose finite candidate does not fit.
-template <uint32_t Q, uint32_t NB>
-struct PackedDist {
-    static constexpr uint32_t QPL = Q, BITS = NB, FM = (1u << NB) - 1u;
-    __device__ static uint32_t f(uint32_t x, int i) { return (x >> (BITS * i)) & FM; }
-    __device__ static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
-        uint32_t c = 0xffffffffu;
-#pragma unroll
-        for (int i = 0; i < (int)QPL; ++i) {
-            const uint32_t l = f(lv, i);
-            if (!((nib >> i) & 1u) || l == FM) continue;
-            const uint64_t sm = (uint64_t)l + w;
-            if (sm < FM) c = (c & ~(FM << (BITS * i))) | ((uint32_t)sm << (BITS * i));
-            else if (sm < INF) ov |= 1u << i;  // finite, but not representable
-        }
-        return c;
-    }
-    __device__ static uint32_t merge(uint32_t cur, uint32_t c) {  // per-label min
-        uint32_t r = 0;
-#pragma unroll
-        for (int i = 0; i < (int)QPL; ++i) r |= min(f(cur, i), f(c, i)) << (BITS * i);
-        return r;
-    }
-    __device__ static uint32_t improved(uint32_t cur, uint32_t c) {
-        uint32_t m = 0;
-#pragma unroll
-        for (int i = 0; i < (int)QPL; ++i) m |= (f(c, i) < f(cur, i) ? 1u : 0u) << i;
-        return m;
-    }
-    __device__ static uint32_t unreached(uint32_t cur) {  // labels still INF
-        uint32_t m = 0;
-#pragma unroll
-        for (int i = 0; i < (int)QPL; ++i) m |= (f(cur, i) == FM ? 1u : 0u) << i;
-        return m;
-    }
-};
+// 16-bit distances, two per word (bytes8.hpp).
 template <>
-struct Packed<LM_D16> : PackedDist<2, 16> {};
+struct Packed<LM_D16> : Bytes16Dist {};
 // 8-bit distances, four per word (bytes8.hpp).
 template <>
 struct Packed<LM_D8> : Bytes8Dist {};

@@ -72,4 +72,27 @@ struct Bytes8Dist {
     EEG_HD static uint32_t unreached(uint32_t cur) { return pack4(zero_bytes(~cur)); }  // labels at INF
 };
 
+// Distances of 16 bits (0xffff = INF), two per word, any weight: the same results as a loop
+// over the two halves (a pending finite half whose sum reaches 0xffff sets its bit of `ov`;
+// weights are clamped at 0xffff, which keeps such sums at or past 0xffff), without branches.
+struct Bytes16Dist {
+    static constexpr uint32_t QPL = 2, BITS = 16, FM = 0xffffu;
+    EEG_HD static uint32_t cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
+        const uint32_t wc = w < 0xffffu ? w : 0xffffu;
+        const uint32_t lo = (nib & 1u) ? (lv & 0xffffu) : 0xffffu, hi = (nib & 2u) ? (lv >> 16) : 0xffffu;
+        const uint32_t slo = lo + wc, shi = hi + wc;  // at most 2 x 0xffff: no carry out
+        const bool flo = lo != 0xffffu, fhi = hi != 0xffffu;
+        ov |= ((flo && slo >= 0xffffu) ? 1u : 0u) | ((fhi && shi >= 0xffffu) ? 2u : 0u);
+        const uint32_t clo = slo < 0xffffu ? slo : 0xffffu, chi = shi < 0xffffu ? shi : 0xffffu;
+        return clo | (chi << 16);
+    }
+    EEG_HD static uint32_t merge(uint32_t cur, uint32_t c) { return pk_min_u16(cur, c); }  // per-label min
+    EEG_HD static uint32_t improved(uint32_t cur, uint32_t c) {  // labels where c beats cur
+        return ((c & 0xffffu) < (cur & 0xffffu) ? 1u : 0u) | ((c >> 16) < (cur >> 16) ? 2u : 0u);
+    }
+    EEG_HD static uint32_t unreached(uint32_t cur) {  // labels at INF
+        return ((cur & 0xffffu) == 0xffffu ? 1u : 0u) | ((cur >> 16) == 0xffffu ? 2u : 0u);
+    }
+};
+
 }  // namespace eeg

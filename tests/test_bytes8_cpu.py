@@ -1,7 +1,8 @@
-"""The bytewise 8-bit label arithmetic of the narrow label planes (eegraph_amd/csrc/bytes8.hpp)
+"""The branch-free label arithmetic of the narrow label planes (eegraph_amd/csrc/bytes8.hpp)
 equals per-label loops: candidates, overflow bits, min/max merges, improved and unreached
-masks, on random words (labels biased to 0, 1, 127, 128, 253, 254 and 255) and weights
-0..254.  The header is compiled for the host with the ROCm clang."""
+masks, on random words (8-bit labels biased to 0, 1, 127, 128, 253, 254 and 255, weights
+0..254; 16-bit labels biased to the ends of their range, weights up to 2^32 - 1).  The header
+is compiled for the host with the ROCm clang."""
 import os
 import shutil
 import subprocess
@@ -41,6 +42,39 @@ static uint32_t d_cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
 static uint32_t d_merge(uint32_t a, uint32_t b) { uint32_t r = 0; for (int i = 0; i < 4; ++i) r |= std::min(f(a, i), f(b, i)) << (8 * i); return r; }
 static uint32_t d_imp(uint32_t cur, uint32_t c) { uint32_t m = 0; for (int i = 0; i < 4; ++i) m |= (f(c, i) < f(cur, i) ? 1u : 0u) << i; return m; }
 static uint32_t d_unr(uint32_t cur) { uint32_t m = 0; for (int i = 0; i < 4; ++i) m |= (f(cur, i) == 0xffu ? 1u : 0u) << i; return m; }
+// 16-bit distances: per-half loop (65535 = INF; a pending finite half whose sum reaches 65535
+// is dropped and flagged)
+static uint32_t h(uint32_t x, int i) { return (x >> (16 * i)) & 0xffffu; }
+static uint32_t e_cand(uint32_t lv, uint32_t w, uint32_t nib, uint32_t& ov) {
+    uint32_t c = 0xffffffffu;
+    for (int i = 0; i < 2; ++i) {
+        const uint32_t l = h(lv, i);
+        if (!((nib >> i) & 1u) || l == 0xffffu) continue;
+        const uint64_t sm = (uint64_t)l + w;
+        if (sm < 0xffffu) c = (c & ~(0xffffu << (16 * i))) | ((uint32_t)sm << (16 * i));
+        else ov |= 1u << i;
+    }
+    return c;
+}
+static uint32_t e_merge(uint32_t a, uint32_t b) { uint32_t r = 0; for (int i = 0; i < 2; ++i) r |= std::min(h(a, i), h(b, i)) << (16 * i); return r; }
+static uint32_t e_imp(uint32_t cur, uint32_t c) { uint32_t m = 0; for (int i = 0; i < 2; ++i) m |= (h(c, i) < h(cur, i) ? 1u : 0u) << i; return m; }
+static uint32_t e_unr(uint32_t cur) { uint32_t m = 0; for (int i = 0; i < 2; ++i) m |= (h(cur, i) == 0xffffu ? 1u : 0u) << i; return m; }
+static long check16(std::mt19937_64& rng) {
+    const uint32_t sp[] = {0, 1, 0x7fff, 0x8000, 0xfffd, 0xfffe, 0xffff};
+    const uint32_t wsp[] = {0, 1, 127, 128, 1000, 0xfffe, 0xffff, 0x10000, 0xffffffffu};
+    auto half = [&]() -> uint32_t { return (rng() & 1) ? sp[rng() % 7] : (uint32_t)(rng() & 0xffff); };
+    long bad = 0;
+    for (long it = 0; it < 2000000; ++it) {
+        const uint32_t lv = half() | (half() << 16), cur = half() | (half() << 16), nib = rng() & 3;
+        const uint32_t w = (rng() & 1) ? (uint32_t)(rng() & 0xffff) : wsp[rng() % 9];
+        uint32_t o1 = 0, o2 = 0;
+        const uint32_t c1 = Bytes16Dist::cand(lv, w, nib, o1), c2 = e_cand(lv, w, nib, o2);
+        const bool ok = c1 == c2 && o1 == o2 && Bytes16Dist::merge(cur, c1) == e_merge(cur, c2) &&
+                        Bytes16Dist::improved(cur, c1) == e_imp(cur, c2) && Bytes16Dist::unreached(cur) == e_unr(cur);
+        if (!ok && bad++ < 5) printf("16-bit mismatch lv=%08x cur=%08x nib=%x w=%u\n", lv, cur, nib, w);
+    }
+    return bad;
+}
 int main() {
     std::mt19937_64 rng(7);
     const uint32_t special[] = {0, 1, 127, 128, 253, 254, 255};
@@ -59,6 +93,7 @@ int main() {
                         Bytes8Widest::improved(cur, a1) == w_imp(cur, a2);
         if (!ok && bad++ < 5) printf("mismatch lv=%08x cur=%08x nib=%x w=%u\n", lv, cur, nib, w);
     }
+    bad += check16(rng);
     printf("mismatches %ld\n", bad);
     return bad != 0;
 }

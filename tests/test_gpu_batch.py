@@ -301,3 +301,23 @@ def test_batch_label_bits_must_be_0_or_32(rmat13):
     n, s, d, w, g = rmat13
     with pytest.raises(eg.EEGError):
         g.run_batch("sssp", np.arange(20, dtype=np.uint32), label_bits=16)
+
+
+@pytest.mark.parametrize("nsrc", [17, 64])
+def test_batch_16_bit_distances_heavy_weights(nsrc):
+    """Weights above 127 take 16-bit distance planes: RMAT-13 with weights 1..1000, answers equal
+    the 32-bit planes' and the oracle's."""
+    s, d, w = eg.gen_rmat(13, 16, seed=9, wmin=1, wmax=1000)
+    n = 1 << 13
+    g = eg.Graph.from_edges(s, d, w, n=n)
+    try:
+        srcs = np.random.default_rng(nsrc).integers(0, n, nsrc).astype(np.uint32)
+        srcs[0] = int(np.argmax(np.bincount(s, minlength=n)))
+        out, st = g.run_batch("sssp", srcs)
+        assert st["label_bits"] == 16
+        wide, _ = g.run_batch("sssp", srcs, label_bits=32)
+        assert np.array_equal(out, wide)
+        for q in (0, 1, nsrc - 1):
+            assert np.array_equal(out[q], O.reference_labels(O.SSSP, n, s, d, w, int(srcs[q]))), q
+    finally:
+        g.close()
